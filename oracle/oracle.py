@@ -1,0 +1,254 @@
+"""ctypes binding of the CPU oracle (oracle/tp_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker / CPU baseline.  The product
+package never imports this module.
+
+Field and descriptor structs mirror oracle/tp_oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libtporacle.so")
+
+FIELD_ZERO, FIELD_UNIFORM, FIELD_AFFINE, FIELD_DIPOLE, FIELD_GRID, FIELD_UXB = 0, 1, 2, 3, 4, 5
+FIELD_KAT_FRONT, FIELD_KAT_SWITCH_ON, FIELD_KAT_NAN_OUTSIDE = 100, 101, 102
+EXTRAP_FILL_NAN, EXTRAP_CLAMP, EXTRAP_PERIODIC = 0, 1, 2
+ALG_BORIS, ALG_MULTISTEP = 0, 1
+FORM_REFERENCE, FORM_RATIONAL = 0, 1
+BND_NAN, BND_SPHERE_IN, BND_SPHERE_OUT, BND_BOX = 1, 2, 4, 8
+RET_SUCCESS, RET_TERMINATED, RET_MAXITERS = 0, 1, 2
+ERR_NAMES = {1: "time step dt is too small", 2: "nt exceeds maxiters", 3: "invalid argument"}
+
+
+class OrcField(C.Structure):
+    pass
+
+
+OrcField._fields_ = [
+    ("kind", C.c_int), ("extrap", C.c_int), ("dtype", C.c_int), ("pad_", C.c_int),
+    ("v", C.c_double * 3), ("jac", C.c_double * 9), ("ft", C.c_double * 3),
+    ("b0", C.c_double), ("radius", C.c_double), ("t_on", C.c_double),
+    ("data", C.c_void_p), ("nx", C.c_long), ("ny", C.c_long), ("nz", C.c_long),
+    ("origin", C.c_double * 3), ("spacing", C.c_double * 3),
+    ("src", C.POINTER(OrcField)),
+]
+
+
+class OrcDesc(C.Structure):
+    _fields_ = [
+        ("alg", C.c_int), ("adaptive", C.c_int), ("order_N", C.c_int), ("nsub", C.c_int),
+        ("form", C.c_int), ("save_start", C.c_int), ("save_end", C.c_int),
+        ("save_everystep", C.c_int), ("bnd_flags", C.c_int), ("pad_", C.c_int),
+        ("safety", C.c_double), ("q2m", C.c_double),
+        ("t0", C.c_double), ("t1", C.c_double), ("dt", C.c_double),
+        ("maxiters", C.c_long), ("savestepinterval", C.c_long), ("max_steps", C.c_long),
+        ("nout_cap", C.c_long), ("r_in", C.c_double), ("r_out", C.c_double),
+        ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
+        ("dt_pp", C.POINTER(C.c_double)), ("nsteps_pp", C.c_long),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        d3 = C.POINTER(C.c_double)
+        L.orc_boris_velocity_update.argtypes = [d3, d3, d3, C.c_double, d3]
+        L.orc_multistep_velocity.argtypes = [d3, d3, d3, C.c_double, C.c_double, C.c_int,
+                                             C.c_int, C.c_int, d3]
+        L.orc_multistep_coeffs.argtypes = [C.c_double, C.c_int, C.c_int, d3]
+        L.orc_rational_polys.argtypes = [C.c_int, d3, d3, d3, d3, d3, C.POINTER(C.c_int)]
+        L.orc_field_eval.argtypes = [C.POINTER(OrcField), d3, C.c_double, d3]
+        L.orc_calculate_nout.argtypes = [C.c_long, C.c_long, C.c_int, C.c_int, C.c_int]
+        L.orc_calculate_nout.restype = C.c_long
+        L.orc_prepare_checks.argtypes = [C.POINTER(OrcDesc), C.POINTER(C.c_long),
+                                         C.POINTER(C.c_long)]
+        L.orc_solve.argtypes = [C.POINTER(OrcDesc), C.POINTER(OrcField), C.POINTER(OrcField),
+                                d3, C.c_long, d3, d3, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.c_int, d3]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ---------------------------------------------------------------------------
+# Field constructors (keep numpy buffers alive on the returned object)
+# ---------------------------------------------------------------------------
+class Field:
+    def __init__(self, st: OrcField, keep=()):
+        self.st = st
+        self.keep = keep
+
+
+def zero() -> Field:
+    return Field(OrcField(kind=FIELD_ZERO))
+
+
+def uniform(v) -> Field:
+    st = OrcField(kind=FIELD_UNIFORM)
+    st.v[:] = [float(a) for a in v]
+    return Field(st)
+
+
+def affine(f0, jac=None, ft=None) -> Field:
+    st = OrcField(kind=FIELD_AFFINE)
+    st.v[:] = [float(a) for a in f0]
+    st.jac[:] = [float(a) for a in (np.zeros(9) if jac is None else np.ravel(jac))]
+    st.ft[:] = [float(a) for a in (np.zeros(3) if ft is None else ft)]
+    return Field(st)
+
+
+def dipole(b0: float, radius: float) -> Field:
+    return Field(OrcField(kind=FIELD_DIPOLE, b0=b0, radius=radius))
+
+
+def grid(data: np.ndarray, origin, spacing, extrap: int) -> Field:
+    """data: array of shape (nz, ny, nx, 3) (C order), i.e. Julia's
+    3 x nx x ny x nz column-major array."""
+    assert data.ndim == 4 and data.shape[3] == 3
+    assert data.dtype in (np.float64, np.float32)
+    data = np.ascontiguousarray(data)
+    nz, ny, nx, _ = data.shape
+    st = OrcField(kind=FIELD_GRID, extrap=extrap, dtype=0 if data.dtype == np.float64 else 1,
+                  data=data.ctypes.data, nx=nx, ny=ny, nz=nz)
+    st.origin[:] = [float(a) for a in origin]
+    st.spacing[:] = [float(a) for a in spacing]
+    return Field(st, keep=(data,))
+
+
+def uxb(u, B: Field) -> Field:
+    st = OrcField(kind=FIELD_UXB)
+    st.v[:] = [float(a) for a in u]
+    st.src = C.pointer(B.st)
+    return Field(st, keep=(B,))
+
+
+def kat(kind: int, v=(0.0, 0.0, 0.0), radius: float = 0.0, t_on: float = 0.0) -> Field:
+    st = OrcField(kind=kind, radius=radius, t_on=t_on)
+    st.v[:] = [float(a) for a in v]
+    return Field(st)
+
+
+def field_eval(f: Field, x, t: float = 0.0) -> np.ndarray:
+    xa = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_field_eval(C.byref(f.st), _dp(xa), float(t), _dp(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Math kernels
+# ---------------------------------------------------------------------------
+def boris_velocity_update(v, E, B, h) -> np.ndarray:
+    va, Ea, Ba = (np.ascontiguousarray(a, dtype=np.float64) for a in (v, E, B))
+    out = np.zeros(3)
+    lib().orc_boris_velocity_update(_dp(va), _dp(Ea), _dp(Ba), float(h), _dp(out))
+    return out
+
+
+def multistep_velocity(v, E, B, q2m, dt, n, N, form=FORM_REFERENCE) -> np.ndarray:
+    va, Ea, Ba = (np.ascontiguousarray(a, dtype=np.float64) for a in (v, E, B))
+    out = np.zeros(3)
+    lib().orc_multistep_velocity(_dp(va), _dp(Ea), _dp(Ba), float(q2m), float(dt), int(n),
+                                 int(N), int(form), _dp(out))
+    return out
+
+
+def multistep_coeffs(tau2: float, n: int, form: int) -> np.ndarray:
+    c = np.zeros(4)
+    lib().orc_multistep_coeffs(float(tau2), int(n), int(form), _dp(c))
+    return c
+
+
+def rational_polys(n: int):
+    arrs = [np.zeros(16) for _ in range(5)]
+    deg = C.c_int(0)
+    rc = lib().orc_rational_polys(int(n), *[_dp(a) for a in arrs], C.byref(deg))
+    if rc:
+        raise ValueError(f"rational form supports 1 <= n <= 12, got {n}")
+    d = deg.value
+    D, P1, P2, P3, P6 = arrs
+    return D[:d + 1], P1[:d + 1], P2[:d], P3[:d], P6[:d]
+
+
+def calculate_nout(nt, K, save_start=True, save_end=True, save_everystep=True) -> int:
+    return int(lib().orc_calculate_nout(int(nt), int(K), int(save_start), int(save_end),
+                                        int(save_everystep)))
+
+
+# ---------------------------------------------------------------------------
+# Solve
+# ---------------------------------------------------------------------------
+class OracleResult:
+    def __init__(self, u, t, length, retcode, elapsed):
+        self.u = u            # (N, cap, 6)
+        self.t = t            # (N, cap)
+        self.len = length     # (N,)
+        self.retcode = retcode
+        self.elapsed = elapsed
+
+    def traj(self, i):
+        n = min(int(self.len[i]), self.u.shape[1])
+        return self.t[i, :n], self.u[i, :n]
+
+
+def solve(xv0, E: Field, B: Field, *, q2m, tspan, dt=0.0, alg=ALG_BORIS, adaptive=False,
+          order_N=2, nsub=1, form=FORM_REFERENCE, safety=0.1, savestepinterval=1,
+          save_start=True, save_end=True, save_everystep=True, maxiters=1_000_000,
+          max_steps=0, nout_cap=None, bnd_flags=0, r_in=0.0, r_out=0.0,
+          box_lo=(-np.inf,) * 3, box_hi=(np.inf,) * 3, dt_pp=None, nsteps_pp=0,
+          nthreads=1) -> OracleResult:
+    xv0 = np.ascontiguousarray(np.atleast_2d(xv0), dtype=np.float64)
+    N = xv0.shape[0]
+    d = OrcDesc(alg=alg, adaptive=int(adaptive), order_N=order_N, nsub=nsub, form=form,
+                save_start=int(save_start), save_end=int(save_end),
+                save_everystep=int(save_everystep), bnd_flags=bnd_flags, safety=safety,
+                q2m=q2m, t0=tspan[0], t1=tspan[1], dt=dt, maxiters=maxiters,
+                savestepinterval=savestepinterval, max_steps=max_steps, r_in=r_in,
+                r_out=r_out, nsteps_pp=nsteps_pp)
+    d.box_lo[:] = [float(a) for a in box_lo]
+    d.box_hi[:] = [float(a) for a in box_hi]
+    dtpp = None
+    if dt_pp is not None:
+        dtpp = np.ascontiguousarray(dt_pp, dtype=np.float64)
+        d.dt_pp = _dp(dtpp)
+    L = lib()
+    if nout_cap is None:
+        if adaptive:
+            raise ValueError("adaptive solve needs nout_cap")
+        nt, nout = C.c_long(0), C.c_long(0)
+        rc = L.orc_prepare_checks(C.byref(d), C.byref(nt), C.byref(nout))
+        if rc:
+            raise ValueError(ERR_NAMES.get(rc, str(rc)))
+        nout_cap = max(int(nout.value), 1)
+    d.nout_cap = nout_cap
+    out_u = np.full((N, nout_cap, 6), np.nan)
+    out_t = np.full((N, nout_cap), np.nan)
+    ln = np.zeros(N, dtype=np.int32)
+    rc_arr = np.zeros(N, dtype=np.int32)
+    el = np.zeros(1)
+    rc = L.orc_solve(C.byref(d), C.byref(E.st), C.byref(B.st), _dp(xv0), N, _dp(out_u),
+                     _dp(out_t), ln.ctypes.data_as(C.POINTER(C.c_int)),
+                     rc_arr.ctypes.data_as(C.POINTER(C.c_int)), int(nthreads), _dp(el))
+    if rc:
+        raise ValueError(ERR_NAMES.get(rc, str(rc)))
+    return OracleResult(out_u, out_t, ln, rc_arr, float(el[0]))

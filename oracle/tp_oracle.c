@@ -1,0 +1,618 @@
+/*
+ * tp_oracle.c -- CPU oracle for the ensemble Boris pusher.
+ *
+ * TEST INFRASTRUCTURE ONLY (see tp_oracle.h).  A plain-C restatement of the
+ * reference TestParticle.jl v0.23.2; each function cites the file:line it
+ * follows.  Compiled with -ffp-contract=off: a product is rounded before it
+ * is added unless the reference's @muladd turns a *scalar* a*b+c into a
+ * muladd (which Julia lowers to a fused multiply-add on FMA hardware); those
+ * sites call fma() explicitly and are marked "@muladd scalar".  Vector
+ * (SVector) expressions are never fused: MuladdMacro's muladd(x, y, z) on
+ * SVectors falls back to x*y + z.
+ *
+ * Rounding conventions that the reference leaves to its dependencies
+ * (FastInterpolations.linear_interp, Base/libm atan and sincos) are fixed
+ * here and shared with the device library; tests pin them against the
+ * reference's known answers and against closed forms.
+ */
+#include "tp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ----------------------------------------------------------------------------
+ * Small vector helpers with StaticArrays' evaluation order.
+ * cross: StaticArrays cross(a, b) = (a2 b3 - a3 b2, a3 b1 - a1 b3, a1 b2 - a2 b1)
+ * sum(abs2, a) and dot(a, b): left fold ((a1 b1 + a2 b2) + a3 b3).
+ * ------------------------------------------------------------------------- */
+static inline void cross3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double dot3(const double a[3], const double b[3])
+{
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline double sumsq3(const double a[3]) { return dot3(a, a); }
+
+/* boris_velocity_update, src/boris/boris.jl:19-31 */
+void orc_boris_velocity_update(const double v[3], const double E[3],
+                               const double B[3], double h, double out[3])
+{
+    double t[3], s[3], vm[3], vmt[3], vp[3], vps[3];
+    for (int c = 0; c < 3; ++c) t[c] = h * B[c];                /* :20 */
+    const double tm2 = sumsq3(t);                                /* :21 */
+    const double den = 1.0 + tm2;
+    for (int c = 0; c < 3; ++c) s[c] = (2.0 * t[c]) / den;       /* :22 */
+    for (int c = 0; c < 3; ++c) vm[c] = v[c] + h * E[c];         /* :24 */
+    cross3(vm, t, vmt);
+    for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vmt[c];          /* :25 */
+    cross3(vp, s, vps);
+    for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vps[c];          /* :26 */
+    for (int c = 0; c < 3; ++c) out[c] = vp[c] + h * E[c];       /* :28 */
+}
+
+/* ----------------------------------------------------------------------------
+ * Multistep / Hyper Boris coefficients, src/boris/multistep_boris.jl:94-123.
+ *
+ * REFERENCE form: atan + sincos with the Taylor branch below
+ * TN_MAG_THRESHOLD = 1e-4 (boris.jl:3).
+ *
+ * RATIONAL form: with alpha = atan(tau), (1 + i tau)^(2n) = (1+u)^n e^{2 i n alpha},
+ * u = tau^2, so every coefficient is a ratio of integer polynomials in u:
+ *   D  = (1+u)^n
+ *   c1 = cos(2 n alpha)            = P1/D,  P1 = Re (1+i tau)^(2n)
+ *   c2 = sin(2 n alpha)/tau        = P2/D,  P2 = Im (1+i tau)^(2n) / tau
+ *   c3 = 2 sin^2(n alpha)/tau^2    = P3/D,  P3 = (D - P1)/u
+ *   c6 = (2n - c2)/tau^2           = P6/D,  P6 = (2n D - P2)/u
+ * The divisions by u are exact polynomial divisions (the constant terms
+ * cancel), so the rational form needs no Taylor branch and no
+ * transcendental; at tau -> 0 it reduces to the branch values of :101-108.
+ * Horner steps use fma; the four ratios share one reciprocal of D.
+ * ------------------------------------------------------------------------- */
+#define ORC_RATIONAL_NMAX 12
+
+static double binom(int n, int k)
+{
+    double r = 1.0;
+    for (int j = 1; j <= k; ++j) r = r * (double)(n - k + j) / (double)j;
+    return floor(r + 0.5);
+}
+
+int orc_rational_polys(int n, double* D, double* P1, double* P2, double* P3,
+                       double* P6, int* deg)
+{
+    if (n < 1 || n > ORC_RATIONAL_NMAX) return ORC_ERR_ARG;
+    const int m = 2 * n;
+    /* (1 + i tau)^m = sum_k C(m,k) i^k tau^k */
+    for (int j = 0; j <= n; ++j) { D[j] = 0; P1[j] = 0; P2[j] = 0; P3[j] = 0; P6[j] = 0; }
+    for (int j = 0; j <= n; ++j) D[j] = binom(n, j);
+    for (int k = 0; k <= m; ++k) {
+        const double b = binom(m, k);
+        if (k % 2 == 0) {
+            const double sg = ((k / 2) % 2 == 0) ? 1.0 : -1.0;
+            P1[k / 2] += sg * b;
+        } else {
+            const double sg = (((k - 1) / 2) % 2 == 0) ? 1.0 : -1.0;
+            P2[(k - 1) / 2] += sg * b;
+        }
+    }
+    /* P3 = (D - P1)/u, P6 = (2n D - P2)/u: drop the (zero) constant term */
+    for (int j = 1; j <= n; ++j) P3[j - 1] = D[j] - P1[j];
+    for (int j = 1; j <= n; ++j) P6[j - 1] = 2.0 * n * D[j] - (j <= n - 1 ? P2[j] : 0.0);
+    *deg = n;   /* D, P1: degree n; P2, P3, P6: degree n-1 */
+    return ORC_OK;
+}
+
+static inline double horner(const double* p, int d, double u)
+{
+    double r = p[d];
+    for (int j = d - 1; j >= 0; --j) r = fma(r, u, p[j]);
+    return r;
+}
+
+void orc_multistep_coeffs(double tau2, int n, int form, double c[4])
+{
+    if (form == ORC_FORM_RATIONAL) {
+        double D[ORC_RATIONAL_NMAX + 1], P1[ORC_RATIONAL_NMAX + 1], P2[ORC_RATIONAL_NMAX + 1],
+            P3[ORC_RATIONAL_NMAX + 1], P6[ORC_RATIONAL_NMAX + 1];
+        int deg = 0;
+        if (orc_rational_polys(n, D, P1, P2, P3, P6, &deg) != ORC_OK) {
+            c[0] = c[1] = c[2] = c[3] = NAN;
+            return;
+        }
+        const double rD = 1.0 / horner(D, deg, tau2);
+        c[0] = horner(P1, deg, tau2) * rD;
+        c[1] = horner(P2, deg - 1, tau2) * rD;
+        c[2] = horner(P3, deg - 1, tau2) * rD;
+        c[3] = horner(P6, deg - 1, tau2) * rD;
+        return;
+    }
+    /* REFERENCE form, multistep_boris.jl:94-120 */
+    const double tmag = sqrt(tau2);                                    /* :95 */
+    const double nn = (double)n;
+    if (tmag < 1.0e-4) {                                               /* :99 */
+        c[0] = fma(-(2.0 * nn * nn), tau2, 1.0);                       /* :101 @muladd scalar */
+        const double n1 = 2.0 * nn, n3 = 4.0 * nn * nn * nn;           /* :103-104 */
+        c[1] = fma(-((n1 + n3) / 3.0), tau2, n1);                      /* :106 @muladd scalar */
+        c[2] = fma(-((4.0 * nn * nn + 2.0 * nn * nn * nn * nn) / 3.0), tau2,
+                   2.0 * nn * nn);                                     /* :107 @muladd scalar */
+        c[3] = (n1 + n3) / 3.0;                                        /* :108 */
+    } else {
+        const double a = atan(tmag);                                   /* :110 */
+        double s, co;
+        s = sin(nn * a);                                               /* :112 */
+        co = cos(nn * a);
+        const double s2 = 2.0 * s * co;                                /* :113 */
+        const double c2 = fma(co, co, -(s * s));                       /* :114 @muladd scalar */
+        c[0] = c2;                                                     /* :116 */
+        c[1] = s2 / tmag;                                              /* :117 */
+        c[2] = 2.0 * s * s / tau2;                                     /* :118 */
+        c[3] = (2.0 * nn - c[1]) / tau2;                               /* :119 */
+    }
+}
+
+/* update_velocity_multistep, src/boris/multistep_boris.jl:65-142 */
+void orc_multistep_velocity(const double v[3], const double E[3], const double B[3],
+                            double q2m, double dt, int n, int N, int form,
+                            double out[3])
+{
+    const double factor = q2m * dt / (2.0 * n);                        /* :73 */
+    double tn[3], en[3];
+    for (int c = 0; c < 3; ++c) tn[c] = factor * B[c];                 /* :75 */
+    for (int c = 0; c < 3; ++c) en[c] = factor * E[c];                 /* :76 */
+    if (N != 2) {                                                      /* :79-92 */
+        const double tm2 = sumsq3(tn);
+        double fN, ecf;
+        if (N == 4) {
+            fN = 1.0 + tm2 / 3.0;
+            ecf = -1.0 / 3.0;
+        } else {
+            fN = (1.0 + tm2 / 3.0) + 2.0 * tm2 * tm2 / 15.0;
+            ecf = -1.0 / 3.0 - 2.0 * tm2 / 15.0;
+        }
+        const double edt = dot3(en, tn);
+        for (int c = 0; c < 3; ++c) en[c] = fN * en[c] + (ecf * edt) * tn[c];
+        for (int c = 0; c < 3; ++c) tn[c] = fN * tn[c];
+    }
+    const double tau2 = sumsq3(tn);                                    /* :94 */
+    double cf[4];
+    orc_multistep_coeffs(tau2, n, form, cf);
+    const double c1 = cf[0], c2 = cf[1], c3 = cf[2], c6 = cf[3];
+    const double c4 = c2, c5 = c3;                                     /* :122-123 */
+    const double vdt = dot3(v, tn), edt = dot3(en, tn);                /* :125-126 */
+    double vxt[3], ext[3];
+    cross3(v, tn, vxt);                                                /* :128 */
+    cross3(en, tn, ext);                                               /* :129 */
+    for (int c = 0; c < 3; ++c) {                                      /* :134-139 */
+        double r = c1 * v[c];
+        r = r + c2 * vxt[c];
+        r = r + (c3 * vdt) * tn[c];
+        r = r + c4 * en[c];
+        r = r + c5 * ext[c];
+        r = r + (c6 * edt) * tn[c];
+        out[c] = r;
+    }
+}
+
+/* ----------------------------------------------------------------------------
+ * Fields.  Trilinear interpolation on a uniform Cartesian grid restates
+ * build_interpolator(CartesianGrid, A::4D, ...) -> FastInterpolations
+ * linear_interp (src/utility/interpolation.jl:119-148, :217-235) with the
+ * extrapolation modes of :204-206:
+ *   FILL_NAN : NaN outside [g_1, g_N] in any dimension (prepare.jl default)
+ *   CLAMP    : coordinate clamped to [g_1, g_N]
+ *   PERIODIC : exclusive wrap, period N*h, node N blends with node 1
+ * Cell coordinate s = (x - g_1) * (1/h), clamping done on s; lerp(a, b, w) =
+ * fma(w, b, (1 - w) * a) along x, then y, then z.
+ * ------------------------------------------------------------------------- */
+static inline double lerp(double a, double b, double w) { return fma(w, b, (1.0 - w) * a); }
+
+static inline double grid_node(const orc_field* f, long i, long j, long k, int c)
+{
+    const long idx = 3 * (i + f->nx * (j + f->ny * k)) + c;
+    if (f->dtype == ORC_DTYPE_F32) return (double)((const float*)f->data)[idx];
+    return ((const double*)f->data)[idx];
+}
+
+static void grid_eval(const orc_field* f, const double x[3], double out[3])
+{
+    const long n[3] = {f->nx, f->ny, f->nz};
+    long i0[3], i1[3];
+    double w[3];
+    for (int d = 0; d < 3; ++d) {
+        const double inv_h = 1.0 / f->spacing[d];
+        double s = (x[d] - f->origin[d]) * inv_h;
+        const double smax = (double)(n[d] - 1);
+        /* a NaN coordinate gives NaN; so does a periodic one too far out
+         * to index (|s| >= 4e18 cells) */
+        if (s != s || (f->extrap == ORC_EXTRAP_PERIODIC && !(fabs(s) < 4.0e18))) {
+            out[0] = out[1] = out[2] = NAN;
+            return;
+        }
+        if (f->extrap == ORC_EXTRAP_PERIODIC) {
+            const double fl = floor(s);
+            w[d] = s - fl;
+            long i = (long)fl % n[d];
+            if (i < 0) i += n[d];
+            i0[d] = i;
+            i1[d] = (i + 1 == n[d]) ? 0 : i + 1;
+        } else {
+            if (f->extrap == ORC_EXTRAP_FILL_NAN) {
+                if (!(s >= 0.0 && s <= smax)) {
+                    out[0] = out[1] = out[2] = NAN;
+                    return;
+                }
+            } else {
+                s = s < 0.0 ? 0.0 : s;
+                s = s > smax ? smax : s;
+            }
+            double fl = floor(s);
+            if (fl > smax - 1.0) fl = smax - 1.0;
+            w[d] = s - fl;
+            i0[d] = (long)fl;
+            i1[d] = i0[d] + 1;
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double c00 = lerp(grid_node(f, i0[0], i0[1], i0[2], c), grid_node(f, i1[0], i0[1], i0[2], c), w[0]);
+        const double c10 = lerp(grid_node(f, i0[0], i1[1], i0[2], c), grid_node(f, i1[0], i1[1], i0[2], c), w[0]);
+        const double c01 = lerp(grid_node(f, i0[0], i0[1], i1[2], c), grid_node(f, i1[0], i0[1], i1[2], c), w[0]);
+        const double c11 = lerp(grid_node(f, i0[0], i1[1], i1[2], c), grid_node(f, i1[0], i1[1], i1[2], c), w[0]);
+        const double c0 = lerp(c00, c10, w[1]);
+        const double c1 = lerp(c01, c11, w[1]);
+        out[c] = lerp(c0, c1, w[2]);
+    }
+}
+
+void orc_field_eval(const orc_field* f, const double x[3], double t, double out[3])
+{
+    switch (f->kind) {
+    case ORC_FIELD_ZERO:                       /* zero.jl:35-51 */
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    case ORC_FIELD_UNIFORM:
+        out[0] = f->v[0]; out[1] = f->v[1]; out[2] = f->v[2];
+        return;
+    case ORC_FIELD_AFFINE:
+        for (int c = 0; c < 3; ++c) {
+            double r = f->v[c] + f->jac[3 * c] * x[0];
+            r = r + f->jac[3 * c + 1] * x[1];
+            r = r + f->jac[3 * c + 2] * x[2];
+            out[c] = r + f->ft[c] * t;
+        }
+        return;
+    case ORC_FIELD_DIPOLE: {
+        const double r2 = sumsq3(x);
+        const double r = sqrt(r2);
+        const double r5 = (r2 * r2) * r;
+        const double k = f->b0 * ((f->radius * f->radius) * f->radius);
+        const double fac = k / r5;
+        out[0] = fac * ((-3.0 * x[0]) * x[2]);
+        out[1] = fac * ((-3.0 * x[1]) * x[2]);
+        out[2] = fac * (r2 - (3.0 * x[2]) * x[2]);
+        return;
+    }
+    case ORC_FIELD_GRID:
+        grid_eval(f, x, out);
+        return;
+    case ORC_FIELD_UXB: {                      /* E = -u x B = B x u */
+        double B[3];
+        orc_field_eval(f->src, x, t, B);
+        cross3(B, f->v, out);
+        return;
+    }
+    case ORC_FIELD_KAT_FRONT: {                /* test_boris.jl:12-21 */
+        double bz;
+        if (x[0] > 100.0 * t) bz = 1.0e-8;
+        else if (x[0] < -10.0) bz = 1.0e-8;
+        else bz = 0.0;
+        out[0] = 0.0; out[1] = 0.0; out[2] = bz;
+        return;
+    }
+    case ORC_FIELD_KAT_SWITCH_ON:              /* test_boris.jl:252 */
+        if (t > f->t_on) { out[0] = f->v[0]; out[1] = f->v[1]; out[2] = f->v[2]; }
+        else { out[0] = out[1] = out[2] = 0.0; }
+        return;
+    case ORC_FIELD_KAT_NAN_OUTSIDE:            /* test_boundary.jl:10-26 */
+        if (sqrt(sumsq3(x)) > f->radius) { out[0] = out[1] = out[2] = NAN; }
+        else { out[0] = f->v[0]; out[1] = f->v[1]; out[2] = f->v[2]; }
+        return;
+    default:
+        out[0] = out[1] = out[2] = NAN;
+    }
+}
+
+/* ----------------------------------------------------------------------------
+ * Solve driver.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    const orc_desc* d;
+    const orc_field* E;
+    const orc_field* B;
+} ctx_t;
+
+/* update_velocity (boris.jl:38-45) or MultistepUpdater (multistep_boris.jl:3-9) */
+static void updater(const ctx_t* c, const double v[3], const double x[3], double h,
+                    double t, double out[3])
+{
+    double E[3], B[3];
+    orc_field_eval(c->E, x, t, E);                       /* boris.jl:40 */
+    orc_field_eval(c->B, x, t, B);                       /* boris.jl:41 */
+    if (c->d->alg == ORC_ALG_BORIS) {
+        const double h2 = (c->d->q2m * 0.5) * h;         /* boris.jl:42 */
+        orc_boris_velocity_update(v, E, B, h2, out);
+    } else {
+        orc_multistep_velocity(v, E, B, c->d->q2m, h, c->d->nsub, c->d->order_N,
+                               c->d->form, out);
+    }
+}
+
+/* isoutside(u, p, t) menu; the reference passes an arbitrary predicate
+ * (boris.jl:95, :459; adaptive_boris.jl:168, :369). */
+static int outside(const orc_desc* d, const double x[3], const double v[3])
+{
+    const int f = d->bnd_flags;
+    if (!f) return 0;
+    if (f & ORC_BND_NAN) {
+        for (int c = 0; c < 3; ++c)
+            if (isnan(x[c]) || isnan(v[c])) return 1;
+    }
+    if (f & (ORC_BND_SPHERE_IN | ORC_BND_SPHERE_OUT)) {
+        const double r = sqrt(sumsq3(x));
+        if ((f & ORC_BND_SPHERE_IN) && r < d->r_in) return 1;
+        if ((f & ORC_BND_SPHERE_OUT) && r > d->r_out) return 1;
+    }
+    if (f & ORC_BND_BOX) {
+        for (int c = 0; c < 3; ++c)
+            if (x[c] < d->box_lo[c] || x[c] > d->box_hi[c]) return 1;
+    }
+    return 0;
+}
+
+static inline void record(double* ou, double* ot, long cap, long i, long k,
+                          const double x[3], const double v[3], double t)
+{
+    double* r = ou + (i * cap + k) * 6;
+    r[0] = x[0]; r[1] = x[1]; r[2] = x[2];
+    r[3] = v[0]; r[4] = v[1]; r[5] = v[2];
+    ot[i * cap + k] = t;
+}
+
+/* _calculate_nout, boris.jl:383-403 */
+long orc_calculate_nout(long nt, long K, int save_start, int save_end, int save_everystep)
+{
+    long nout = 0;
+    if (save_start) nout += 1;
+    if (save_everystep) {
+        const long steps = nt / K;
+        const int last_is_step = (nt > 0) && (nt % K == 0);
+        nout += steps;
+        if (!save_end && last_is_step) nout -= 1;
+        if (save_end && !last_is_step) nout += 1;
+    } else if (save_end) {
+        nout += 1;
+    }
+    return nout;
+}
+
+/* _prepare_checks, boris.jl:345-362 (per-particle dt: nt = nsteps_pp) */
+int orc_prepare_checks(const orc_desc* d, long* nt, long* nout)
+{
+    if (d->savestepinterval < 1) return ORC_ERR_ARG;
+    long n;
+    if (d->dt_pp) {
+        n = d->nsteps_pp;
+    } else {
+        if (fabs(d->dt) < 10.0 * 2.220446049250313e-16) return ORC_ERR_DT_TOO_SMALL; /* :349 */
+        const double ttotal = d->t1 - d->t0;                                          /* :352 */
+        n = labs((long)nearbyint(ttotal / d->dt));                                    /* :353 */
+    }
+    if (n > d->maxiters) return ORC_ERR_MAXITERS;                                     /* :354 */
+    *nt = n;
+    *nout = orc_calculate_nout(n, d->savestepinterval, d->save_start, d->save_end,
+                               d->save_everystep);
+    return ORC_OK;
+}
+
+/* _boris_single + _boris_loop!, boris.jl:445-557 */
+static void solve_fixed_one(const ctx_t* c, long i, const double* xv0, long nt, long nout,
+                            double* ou, double* ot, int* olen, int* oret)
+{
+    const orc_desc* d = c->d;
+    const long cap = d->nout_cap;
+    const long K = d->savestepinterval;
+    const double dt = d->dt_pp ? d->dt_pp[i] : d->dt;
+    const double t0 = d->t0;
+    const double t1 = d->dt_pp ? t0 + (double)nt * dt : d->t1;
+    double x[3] = {xv0[6 * i], xv0[6 * i + 1], xv0[6 * i + 2]};
+    double v[3] = {xv0[6 * i + 3], xv0[6 * i + 4], xv0[6 * i + 5]};
+    double vtmp[3];
+    long iout = 0;
+    double tlast = 0.0;
+
+    if (d->save_start) {                                        /* :507-511 */
+        record(ou, ot, cap, i, iout, x, v, t0);
+        tlast = t0;
+        iout += 1;
+    }
+    updater(c, v, x, -0.5 * dt, t0, vtmp);                      /* :514 */
+    memcpy(v, vtmp, sizeof v);
+
+    /* _boris_loop!, :450-477 */
+    long it = 1;
+    long final_step = nt;
+    double t = fma(-0.5, dt, t0);                               /* :451 @muladd scalar */
+    while (it <= nt) {
+        double vprev[3] = {v[0], v[1], v[2]};
+        t += dt;                                                /* :454 */
+        updater(c, v, x, dt, t, vtmp);                          /* :455 */
+        memcpy(v, vtmp, sizeof v);
+        double xn[3];
+        for (int k = 0; k < 3; ++k) xn[k] = x[k] + v[k] * dt;   /* :457 */
+        if (outside(d, xn, v)) {                                /* :459-461 */
+            final_step = it - 1;
+            memcpy(v, vprev, sizeof v);
+            goto loop_done;
+        }
+        if (d->save_everystep && (it - 1) > 0 && (it - 1) % K == 0) {   /* :463 */
+            iout += 1;
+            if (iout <= nout) {
+                const double tc = fma(-0.5, dt, t);             /* :466 @muladd scalar */
+                updater(c, vprev, x, 0.5 * dt, tc, vtmp);       /* :467 */
+                record(ou, ot, cap, i, iout - 1, x, vtmp, tc);
+                tlast = tc;
+            }
+        }
+        memcpy(x, xn, sizeof x);                                /* :474 */
+        it += 1;
+    }
+    final_step = it - 1;
+loop_done:;
+    /* :522-543 */
+    int should = 0;
+    if (d->save_end) should = 1;
+    else if (d->save_everystep && final_step > 0 && final_step % K == 0) should = 1;
+    if (iout < nout && should) {
+        const double tf = (final_step == nt) ? t1 : fma((double)final_step, dt, t0);
+        if (iout == 0 || tlast < tf) {
+            const double dtf = tf - fma((double)final_step - 0.5, dt, t0);   /* :534 */
+            updater(c, v, x, dtf, tf, vtmp);                                /* :535 */
+            record(ou, ot, cap, i, iout, x, vtmp, tf);
+            iout += 1;
+        }
+    }
+    olen[i] = (int)iout;                                        /* :545-551 */
+    oret[i] = (iout < nout) ? ORC_RET_TERMINATED : ORC_RET_SUCCESS;
+}
+
+/* _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266 */
+static void solve_adaptive_one(const ctx_t* c, long i, const double* xv0, double* ou,
+                               double* ot, int* olen, int* oret)
+{
+    const orc_desc* d = c->d;
+    const long cap = d->nout_cap;
+    const long K = d->savestepinterval;
+    const double t0 = d->t0, t1 = d->t1;
+    const double sg = (t1 - t0) > 0 ? 1.0 : ((t1 - t0) < 0 ? -1.0 : 0.0);
+    const double C = ((2.0 * M_PI) * d->safety * sg) / fabs(d->q2m);       /* :164 */
+    double x[3] = {xv0[6 * i], xv0[6 * i + 1], xv0[6 * i + 2]};
+    double v[3] = {xv0[6 * i + 3], xv0[6 * i + 4], xv0[6 * i + 5]};
+    double vtmp[3], Bv[3];
+    double t = t0;
+    const double ttotal = t1 - t0;
+    long nrec = 0;
+    double tlast = 0.0;
+
+#define PUSH(xx, vv, tt)                                         \
+    do {                                                         \
+        if (nrec < cap) record(ou, ot, cap, i, nrec, xx, vv, tt); \
+        tlast = (tt);                                            \
+        nrec += 1;                                               \
+    } while (0)
+
+    if (d->save_start) PUSH(x, v, t);                              /* :179-186 */
+    orc_field_eval(c->B, x, t, Bv);
+    double dt = C / sqrt(sumsq3(Bv));                               /* :188-189 */
+    updater(c, v, x, -0.5 * dt, t, vtmp);                           /* :192 */
+    memcpy(v, vtmp, sizeof v);
+    long it = 1;
+    int should = d->save_end;
+    int ret = ORC_RET_SUCCESS;
+
+    while (fabs(t - t0) < fabs(ttotal)) {                           /* :197 */
+        if (fabs(t + dt - t0) > fabs(ttotal)) {                     /* :198-203 */
+            const double dts = t1 - t;
+            updater(c, v, x, 0.5 * dt, t, vtmp);
+            updater(c, vtmp, x, -0.5 * dts, t, v);
+            dt = dts;
+        }
+        if (d->save_everystep && (it - 1) > 0 && (it - 1) % K == 0) { /* :205-218 */
+            updater(c, v, x, 0.5 * dt, t, vtmp);
+            PUSH(x, vtmp, t);
+        }
+        const double tmid = t + 0.5 * dt;                           /* :220 */
+        double vn[3], xn[3];
+        updater(c, v, x, dt, tmid, vn);                             /* :221 */
+        for (int k = 0; k < 3; ++k) xn[k] = x[k] + vn[k] * dt;      /* :223 */
+        const double tn = t + dt;                                   /* :224 */
+        if (outside(d, xn, vn)) {                                   /* :227-231 */
+            should = 1;
+            ret = ORC_RET_TERMINATED;
+            break;
+        }
+        memcpy(x, xn, sizeof x);                                    /* :233-235 */
+        t = tn;
+        memcpy(v, vn, sizeof v);
+        orc_field_eval(c->B, x, t, Bv);                             /* :237-238 */
+        const double dtn = C / sqrt(sumsq3(Bv));
+        updater(c, v, x, 0.5 * dt, t, vtmp);                        /* :241-242 */
+        updater(c, vtmp, x, -0.5 * dtn, t, v);
+        dt = dtn;                                                   /* :243 */
+        it += 1;
+        if (d->save_everystep && (it - 1) % K == 0) should = 1;    /* :245-247 */
+        if (d->max_steps > 0 && it - 1 >= d->max_steps) {          /* step cap (extension) */
+            ret = ORC_RET_MAXITERS;
+            break;
+        }
+    }
+    if (should && (nrec == 0 || tlast != t)) {                      /* :250-259 */
+        updater(c, v, x, 0.5 * dt, t, vtmp);
+        PUSH(x, vtmp, t);
+    }
+#undef PUSH
+    olen[i] = (int)nrec;
+    oret[i] = ret;
+}
+
+int orc_solve(const orc_desc* d, const orc_field* E, const orc_field* B,
+              const double* xv0, long N, double* out_u, double* out_t,
+              int* out_len, int* retcode, int nthreads, double* elapsed_s)
+{
+    long nt = 0, nout = 0;
+    if (!d->adaptive) {
+        const int e = orc_prepare_checks(d, &nt, &nout);
+        if (e != ORC_OK) return e;
+        if (d->nout_cap < nout) return ORC_ERR_ARG;
+    } else if (d->savestepinterval < 1) {
+        return ORC_ERR_ARG;
+    }
+    if (d->alg == ORC_ALG_MULTISTEP) {
+        if (d->order_N != 2 && d->order_N != 4 && d->order_N != 6) return ORC_ERR_ARG;
+        if (d->nsub < 1) return ORC_ERR_ARG;
+        if (d->form == ORC_FORM_RATIONAL && d->nsub > ORC_RATIONAL_NMAX) return ORC_ERR_ARG;
+    }
+    ctx_t c = {d, E, B};
+    if (nthreads < 1) nthreads = 1;
+    /* EnsembleThreads: nthreads contiguous chunks (boris.jl:204-212;
+     * ChunkSplitters index_chunks(1:N; n) gives the first N % n chunks one
+     * extra element). */
+#ifdef _OPENMP
+    const double ts = omp_get_wtime();
+#pragma omp parallel for schedule(static, 1) num_threads(nthreads)
+#else
+    const double ts = 0.0;
+#endif
+    for (int ch = 0; ch < nthreads; ++ch) {
+        const long base = N / nthreads, rem = N % nthreads;
+        const long lo = ch * base + (ch < rem ? ch : rem);
+        const long hi = lo + base + (ch < rem ? 1 : 0);
+        for (long i = lo; i < hi; ++i) {
+            if (d->adaptive) solve_adaptive_one(&c, i, xv0, out_u, out_t, out_len, retcode);
+            else solve_fixed_one(&c, i, xv0, nt, nout, out_u, out_t, out_len, retcode);
+        }
+    }
+#ifdef _OPENMP
+    if (elapsed_s) *elapsed_s = omp_get_wtime() - ts;
+#else
+    (void)ts;
+    if (elapsed_s) *elapsed_s = 0.0;
+#endif
+    return ORC_OK;
+}

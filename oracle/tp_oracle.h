@@ -1,0 +1,116 @@
+/*
+ * tp_oracle.h -- CPU oracle for the ensemble Boris pusher.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm (TestParticle.jl v0.23.2, src/boris/{boris,multistep_boris,adaptive_boris}.jl,
+ * src/utility/interpolation.jl).  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only as the checker or the CPU
+ * baseline.  The product (testparticle.jl_amd/) never links, loads or calls it.
+ *
+ * Every function in tp_oracle.c cites the reference file:line it follows.
+ */
+#ifndef TP_ORACLE_H
+#define TP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field kinds.  The first block is the menu the device library supports;
+ * the ORC_FIELD_KAT_* kinds restate closures that appear only in the
+ * reference's own tests (test/test_boris.jl, test/test_boundary.jl). */
+enum {
+    ORC_FIELD_ZERO = 0,      /* src/utility/zero.jl:35-51 */
+    ORC_FIELD_UNIFORM = 1,   /* F = v */
+    ORC_FIELD_AFFINE = 2,    /* F_i = f0_i + sum_j J_ij x_j + ft_i t */
+    ORC_FIELD_DIPOLE = 3,    /* B = b0 R^3 / r^5 (-3xz, -3yz, r^2 - 3z^2) */
+    ORC_FIELD_GRID = 4,      /* trilinear on a uniform Cartesian grid */
+    ORC_FIELD_UXB = 5,       /* E = -u x B(x, t), B = *src */
+    ORC_FIELD_KAT_FRONT = 100,      /* test_boris.jl:12-21 time_varying_B */
+    ORC_FIELD_KAT_SWITCH_ON = 101,  /* test_boris.jl:252 B = t > t_on ? v : 0 */
+    ORC_FIELD_KAT_NAN_OUTSIDE = 102 /* test_boundary.jl:10-26 */
+};
+
+enum { ORC_EXTRAP_FILL_NAN = 0, ORC_EXTRAP_CLAMP = 1, ORC_EXTRAP_PERIODIC = 2 };
+enum { ORC_DTYPE_F64 = 0, ORC_DTYPE_F32 = 1 };
+
+typedef struct orc_field {
+    int kind;
+    int extrap;
+    int dtype;
+    int pad_;
+    double v[3];
+    double jac[9];     /* row-major: F_i += jac[3*i+j] * x_j */
+    double ft[3];
+    double b0;
+    double radius;
+    double t_on;
+    const void* data;  /* 3 x nx x ny x nz, component fastest (Julia layout) */
+    long nx, ny, nz;
+    double origin[3];
+    double spacing[3];
+    const struct orc_field* src;
+} orc_field;
+
+enum { ORC_ALG_BORIS = 0, ORC_ALG_MULTISTEP = 1 };
+enum { ORC_FORM_REFERENCE = 0, ORC_FORM_RATIONAL = 1 };
+enum {
+    ORC_BND_NAN = 1,         /* any(isnan, u) */
+    ORC_BND_SPHERE_IN = 2,   /* norm(x) < r_in   (removed inside) */
+    ORC_BND_SPHERE_OUT = 4,  /* norm(x) > r_out */
+    ORC_BND_BOX = 8          /* x outside [lo, hi] in any component */
+};
+enum { ORC_RET_SUCCESS = 0, ORC_RET_TERMINATED = 1, ORC_RET_MAXITERS = 2 };
+enum { ORC_OK = 0, ORC_ERR_DT_TOO_SMALL = 1, ORC_ERR_MAXITERS = 2, ORC_ERR_ARG = 3 };
+
+typedef struct orc_desc {
+    int alg;
+    int adaptive;
+    int order_N;       /* 2, 4, 6 */
+    int nsub;          /* n >= 1 */
+    int form;          /* ORC_FORM_* (multistep only) */
+    int save_start, save_end, save_everystep;
+    int bnd_flags;
+    int pad_;
+    double safety;
+    double q2m;
+    double t0, t1, dt;
+    long maxiters;
+    long savestepinterval;
+    long max_steps;    /* adaptive step cap, 0 = none (reference has none) */
+    long nout_cap;     /* output records per particle */
+    double r_in, r_out;
+    double box_lo[3], box_hi[3];
+    const double* dt_pp;   /* per-particle dt (fixed step), NULL = use dt */
+    long nsteps_pp;        /* steps per particle when dt_pp != NULL */
+} orc_desc;
+
+/* boris.jl:19-31 */
+void orc_boris_velocity_update(const double v[3], const double E[3],
+                               const double B[3], double qdt_2m, double out[3]);
+/* multistep_boris.jl:65-142 (form selects atan/sincos or rational coefficients) */
+void orc_multistep_velocity(const double v[3], const double E[3], const double B[3],
+                            double q2m, double dt, int n, int N, int form,
+                            double out[3]);
+/* coefficients c1, c2, c3, c6 of multistep_boris.jl:99-120 for |t_n|^2 = tau2 */
+void orc_multistep_coeffs(double tau2, int n, int form, double c[4]);
+/* integer polynomial coefficients of the rational form (see tp_oracle.c) */
+int orc_rational_polys(int n, double* D, double* P1, double* P2, double* P3,
+                       double* P6, int* deg);
+void orc_field_eval(const orc_field* f, const double x[3], double t, double out[3]);
+/* boris.jl:383-403 */
+long orc_calculate_nout(long nt, long K, int save_start, int save_end, int save_everystep);
+/* boris.jl:345-362 */
+int orc_prepare_checks(const orc_desc* d, long* nt, long* nout);
+/* Ensemble solve (EnsembleThreads chunking, boris.jl:189-215 and
+ * adaptive_boris.jl:196-213).  Records: out_u[(i*cap + k)*6 + c],
+ * out_t[i*cap + k]; out_len[i] = records written (may exceed cap for the
+ * adaptive solver, in which case the extra records were dropped). */
+int orc_solve(const orc_desc* d, const orc_field* E, const orc_field* B,
+              const double* xv0, long N, double* out_u, double* out_t,
+              int* out_len, int* retcode, int nthreads, double* elapsed_s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
