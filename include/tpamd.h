@@ -1,0 +1,202 @@
+/*
+ * tpamd.h -- C ABI of the MI355X (gfx950) ensemble Boris pusher.
+ *
+ * Drop-in boundary for TestParticle.jl's ensemble Boris path.  Plain C types
+ * only: pointers, sizes, doubles.  No exceptions cross the boundary; every
+ * entry point returns a tpamd_status and tpamd_last_error() holds the message.
+ *
+ * Layouts follow the Julia caller so that no host-side reshuffle is needed:
+ *   - particle state  xv[N][6]        = Julia 6 x N column-major (x, y, z, vx, vy, vz)
+ *   - saved records   u[N][cap][6]    = Julia 6 x cap x N column-major
+ *   - saved times     t[N][cap]
+ *   - gridded fields  A[nz][ny][nx][3] = Julia 3 x nx x ny x nz column-major
+ *
+ * Ownership: host buffers are caller-owned; device memory is owned by the
+ * context (fields, staging) or by the caller (tpamd_solve_device).
+ * Threading: a context is not thread-safe; calls on one context are
+ * serialised by the caller.  Calls block until results are in the caller's
+ * buffers (tpamd_solve) or the work is complete on the stream
+ * (tpamd_solve_device).
+ */
+#ifndef TPAMD_H
+#define TPAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPAMD_ABI_VERSION 1
+
+typedef struct tpamd_ctx tpamd_ctx;
+typedef struct tpamd_field tpamd_field;
+
+typedef enum tpamd_status {
+    TPAMD_OK = 0,
+    TPAMD_ERR_ARGUMENT = 1,      /* invalid argument (ArgumentError) */
+    TPAMD_ERR_DT_TOO_SMALL = 2,  /* |dt| < 10 eps, src/boris/boris.jl:349-351 */
+    TPAMD_ERR_MAXITERS = 3,      /* nt > maxiters, src/boris/boris.jl:354-356 */
+    TPAMD_ERR_ORDER = 4,         /* MultistepBoris N not in {2,4,6}, src/types.jl:70-75 */
+    TPAMD_ERR_DEVICE = 5,        /* HIP runtime error */
+    TPAMD_ERR_OOM = 6,           /* device allocation failed */
+    TPAMD_ERR_UNSUPPORTED = 7,   /* combination the device path does not provide */
+    TPAMD_ERR_CAPACITY = 8       /* adaptive records exceeded nout_cap (stats->max_len says how many) */
+} tpamd_status;
+
+/* Field kinds: the device-side menu that replaces the Julia field callables
+ * carried in p = (q2m, m, E, B, F) (src/prepare.jl:36-50, src/types.jl:32). */
+typedef enum tpamd_field_kind {
+    TPAMD_FIELD_ZERO = 0,     /* ZeroField, src/utility/zero.jl:1,35-51 */
+    TPAMD_FIELD_UNIFORM = 1,  /* constant vector */
+    TPAMD_FIELD_AFFINE = 2,   /* F_i = f0_i + sum_j J_ij x_j + ft_i t */
+    TPAMD_FIELD_DIPOLE = 3,   /* B = b0 R^3/r^5 (-3xz, -3yz, r^2-3z^2) */
+    TPAMD_FIELD_GRID = 4,     /* FieldInterpolator over a uniform CartesianGrid, order 1 */
+    TPAMD_FIELD_UXB = 5       /* E = -u x B(x), B the solve's magnetic field */
+} tpamd_field_kind;
+
+/* Extrapolation of gridded fields, src/utility/interpolation.jl:203-206 */
+typedef enum tpamd_extrap {
+    TPAMD_EXTRAP_FILL_NAN = 0, /* FillExtrap(NaN), the prepare() default (src/prepare.jl:257-265) */
+    TPAMD_EXTRAP_CLAMP = 1,    /* ClampExtrap() */
+    TPAMD_EXTRAP_PERIODIC = 2  /* WrapExtrap() / PeriodicBC(endpoint=:exclusive): period N*h */
+} tpamd_extrap;
+
+typedef enum tpamd_dtype { TPAMD_F64 = 0, TPAMD_F32 = 1 } tpamd_dtype;
+
+typedef enum tpamd_alg { TPAMD_ALG_BORIS = 0, TPAMD_ALG_MULTISTEP = 1 } tpamd_alg;
+
+/* Multistep coefficient evaluation: RATIONAL evaluates cos/sin(2 n atan tau)
+ * as integer-polynomial ratios in tau^2 (no transcendental, no Taylor
+ * branch); REFERENCE follows src/boris/multistep_boris.jl:94-120 literally. */
+typedef enum tpamd_form { TPAMD_FORM_REFERENCE = 0, TPAMD_FORM_RATIONAL = 1 } tpamd_form;
+
+/* FP64: state, fields and arithmetic in double, rounding-identical to the
+ * CPU restatement of the reference.  FP32C: field storage and gather in
+ * float, Boris rotation in float, positions as a compensated float pair
+ * (hi + lo) with periodic wrap counts, time kept in double. */
+typedef enum tpamd_precision { TPAMD_FP64 = 0, TPAMD_FP32C = 1 } tpamd_precision;
+
+/* isoutside(u, p, t) menu (src/boris/boris.jl:95,457-461;
+ * src/boris/adaptive_boris.jl:26,227-231).  Flags are OR-ed. */
+typedef enum tpamd_boundary {
+    TPAMD_BND_NAN = 1,        /* any(isnan, u) */
+    TPAMD_BND_SPHERE_IN = 2,  /* norm(x) < r_in  (e.g. r < R_E) */
+    TPAMD_BND_SPHERE_OUT = 4, /* norm(x) > r_out */
+    TPAMD_BND_BOX = 8         /* x outside [box_lo, box_hi] in any component */
+} tpamd_boundary;
+
+/* Per-trajectory return codes (SciMLBase ReturnCode) */
+typedef enum tpamd_retcode {
+    TPAMD_RET_SUCCESS = 0,
+    TPAMD_RET_TERMINATED = 1, /* isoutside fired; src/boris/boris.jl:545-551 */
+    TPAMD_RET_MAXITERS = 2    /* adaptive step cap reached (max_steps) */
+} tpamd_retcode;
+
+/* Solve descriptor: the keyword arguments of
+ *   solve(prob, alg, [backend], [ensemblealg]; dt, trajectories, savestepinterval,
+ *         isoutside, save_start, save_end, save_everystep, maxiters)
+ * (src/boris/boris.jl:88-106; src/boris/adaptive_boris.jl:21-33;
+ *  src/boris/boris_kernel.jl:312-317) plus the algorithm struct fields
+ * (src/types.jl:51-107).  Zero-initialise, then set what is needed. */
+typedef struct tpamd_solve_desc {
+    int32_t alg;              /* tpamd_alg */
+    int32_t adaptive;         /* 0: fixed step (Boris{false}); 1: AdaptiveBoris / AdaptiveMultistepBoris */
+    int32_t order_N;          /* MultistepBoris{N}: 2, 4 or 6 */
+    int32_t nsub;             /* MultistepBoris n >= 1 */
+    int32_t form;             /* tpamd_form (multistep only) */
+    int32_t precision;        /* tpamd_precision */
+    int32_t save_start;
+    int32_t save_end;
+    int32_t save_everystep;
+    int32_t bnd_flags;        /* tpamd_boundary bits */
+    double safety;            /* adaptive: dt = safety * 2 pi / |q2m B| */
+    double q2m;               /* p[1] = q/m (src/prepare.jl:211) */
+    double t0, t1;            /* tspan */
+    double dt;                /* fixed step */
+    int64_t maxiters;         /* default 1_000_000 */
+    int64_t savestepinterval; /* K >= 1 */
+    int64_t max_steps;        /* adaptive step cap per particle; 0 = none (reference) */
+    int64_t nout_cap;         /* record capacity per particle (>= tpamd_plan's nout) */
+    double r_in, r_out;
+    double box_lo[3], box_hi[3];
+    const double* dt_pp;      /* optional per-particle dt (fixed step): tspan_i = (t0, t0 + nsteps_pp dt_i) */
+    int64_t nsteps_pp;
+    int32_t block_size;       /* 0 = library default */
+    int32_t reserved[7];
+} tpamd_solve_desc;
+
+typedef struct tpamd_stats {
+    double kernel_ms;         /* device time of the push kernel, HIP events on the solve stream */
+    double total_ms;          /* host wall time of the call */
+    int64_t particle_steps;   /* Boris steps advanced, summed over particles */
+    int64_t nt;               /* fixed-step count (0 for adaptive) */
+    int64_t nout;             /* records per particle the schedule asks for (fixed step) */
+    int64_t max_len;          /* largest record count of any particle */
+} tpamd_stats;
+
+/* ---- library / context -------------------------------------------------- */
+int32_t tpamd_abi_version(void);
+/* Opens a context on HIP device `device` (its own default-priority stream). */
+int32_t tpamd_init(int32_t device, tpamd_ctx** out);
+int32_t tpamd_finalize(tpamd_ctx* ctx);
+/* Message of the last failed call on ctx (or of the last failed tpamd_init when ctx is NULL). */
+const char* tpamd_last_error(const tpamd_ctx* ctx);
+
+/* ---- fields ------------------------------------------------------------- *
+ * Replace prepare(E, B; ...) / prepare(x, y, z, E, B; order=1, bc) and
+ * build_interpolator(CartesianGrid, A::4D, x, y, z, 1, bc)
+ * (src/prepare.jl:93-104,145-153; src/utility/interpolation.jl:217-235).   */
+int32_t tpamd_field_zero(tpamd_ctx* ctx, tpamd_field** out);
+int32_t tpamd_field_uniform(tpamd_ctx* ctx, const double value[3], tpamd_field** out);
+int32_t tpamd_field_affine(tpamd_ctx* ctx, const double f0[3], const double jac[9],
+                           const double ft[3], tpamd_field** out);
+/* Dipole with B(R, equator) = (0, 0, b0); the reference's tests use
+ * Magnetostatics.Dipole(BMoment_Earth) (test/test_gc.jl:180). */
+int32_t tpamd_field_dipole(tpamd_ctx* ctx, double b0, double radius, tpamd_field** out);
+/* Uniform Cartesian grid: node (i,j,k) at origin + (i,j,k) .* spacing.
+ * `data` is a host array of `dtype` in the Julia 3 x nx x ny x nz layout; it
+ * is copied to the device and stored as `storage_dtype`. */
+int32_t tpamd_field_grid(tpamd_ctx* ctx, const void* data, int32_t dtype, int64_t nx,
+                         int64_t ny, int64_t nz, const double origin[3],
+                         const double spacing[3], int32_t extrap, int32_t storage_dtype,
+                         tpamd_field** out);
+/* E = -u x B(x) for a uniform drift u; `B` must be the B field of the solve. */
+int32_t tpamd_field_uxb(tpamd_ctx* ctx, const double u[3], const tpamd_field* B,
+                        tpamd_field** out);
+/* Evaluates a field at n points (host arrays x[n][3] -> out[n][3]) on the device:
+ * the device counterpart of calling the Julia field callable, E(r, t). */
+int32_t tpamd_field_eval(tpamd_ctx* ctx, const tpamd_field* f, const tpamd_field* B,
+                         const double* x, int64_t n, double t, int32_t precision,
+                         double* out);
+int32_t tpamd_field_free(tpamd_ctx* ctx, tpamd_field* f);
+
+/* ---- solve -------------------------------------------------------------- */
+/* _prepare_checks + _calculate_nout (src/boris/boris.jl:345-403): step count
+ * and records per particle of a fixed-step solve; raises the same errors. */
+int32_t tpamd_plan(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int64_t* nt, int64_t* nout);
+
+/* Ensemble solve with host buffers; replaces
+ *   solve(prob, ::Boris{false}, backend::KA.Backend, ensemblealg; ...)
+ *     (src/boris/boris_kernel.jl:312-382)
+ * and the CPU ensemble paths it mirrors
+ *   _solve(::EnsembleSerial/::EnsembleThreads, ...) (src/boris/boris.jl:137-215),
+ *   _solve_adaptive(...) (src/boris/adaptive_boris.jl:40-71).
+ * xv0[N][6] initial states (prob_func already applied by the caller);
+ * out_u[N][cap][6], out_t[N][cap], out_len[N], retcode[N]; cap = desc->nout_cap. */
+int32_t tpamd_solve(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* E,
+                    const tpamd_field* B, const double* xv0, int64_t N, double* out_u,
+                    double* out_t, int32_t* out_len, int32_t* retcode, tpamd_stats* stats);
+
+/* Same solve with device-resident buffers on `stream` (a hipStream_t; NULL =
+ * the context's stream).  desc->dt_pp, if set, is a device pointer.  Returns
+ * after the work on the stream has completed. */
+int32_t tpamd_solve_device(tpamd_ctx* ctx, const tpamd_solve_desc* desc,
+                           const tpamd_field* E, const tpamd_field* B, const double* d_xv0,
+                           int64_t N, double* d_out_u, double* d_out_t, int32_t* d_out_len,
+                           int32_t* d_retcode, void* stream, tpamd_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

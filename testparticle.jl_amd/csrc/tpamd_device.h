@@ -1,0 +1,566 @@
+// tpamd_device.h -- device-side math of the ensemble Boris pusher (gfx950).
+//
+// Two arithmetic policies share one driver:
+//
+//  * FP64 ("exact"): state, fields and arithmetic in double with the
+//    reference's operation order and no contraction (the library is built
+//    with -ffp-contract=off); fma() appears only where the reference's
+//    @muladd fuses a scalar a*b+c.  Results are rounding-identical to the CPU
+//    restatement of src/boris/boris.jl et al., so trajectories agree bit for
+//    bit even in chaotic fields.
+//
+//  * FP32C ("compensated"): gridded fields stored and gathered in float, the
+//    rotation in float with hand-placed fmaf, the position as an unevaluated
+//    float pair hi + lo updated with TwoSum, periodic wrap counts as int, and
+//    the time in double.  Positions are kept relative to the grid origin so
+//    the cell lookup never loses the fraction of a cell.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tpamd {
+
+constexpr int kRationalNMax = 12;
+// adaptive solves without a step cap stop after this many steps per particle
+constexpr long long kAdaptiveHardCap = 1LL << 31;
+
+struct DevField {
+    int kind;        // tpamd_field_kind
+    int extrap;      // tpamd_extrap (grids)
+    int dtype;       // storage dtype of `data` (grids)
+    int pad0;
+    int n[3];
+    int pad1;
+    double v[3];     // uniform value / affine f0 / UxB drift velocity
+    double jac[9];   // affine Jacobian, row-major
+    double ft[3];    // affine time derivative
+    double b0;       // dipole
+    double radius;
+    double origin[3];
+    double inv_h[3]; // 1 / spacing
+    double period[3];// n * spacing (periodic grids)
+    float inv_hf[3];
+    float pad2;
+    const void* data;
+};
+
+struct Poly {
+    double D[kRationalNMax + 1];
+    double P1[kRationalNMax + 1];
+    double P2[kRationalNMax];
+    double P3[kRationalNMax];
+    double P6[kRationalNMax];
+    int deg;
+    int pad;
+};
+
+struct KParams {
+    DevField E, B;
+    Poly poly;
+    int alg, order_N, nsub, form;
+    int save_start, save_end, save_every, bnd_flags;
+    int e_is_uxb, frame_periodic, pad0, pad1;
+    double q2m, t0, t1, dt, C;
+    long long nt, nout, cap, K, max_steps, N;
+    double r_in, r_out;
+    double box_lo[3], box_hi[3];
+    double frame_origin[3];   // FP32C: positions stored relative to this point
+    double frame_period[3];   // FP32C: wrap period (0 = no wrap)
+    const double* xv0;
+    const double* dt_pp;
+    double* out_u;
+    double* out_t;
+    int* out_len;
+    int* retcode;
+    unsigned long long* steps;
+};
+
+enum { kFieldZero = 0, kFieldUniform = 1, kFieldAffine = 2, kFieldDipole = 3, kFieldGrid = 4,
+       kFieldUxB = 5 };
+enum { kExtrapFillNaN = 0, kExtrapClamp = 1, kExtrapPeriodic = 2 };
+enum { kBndNaN = 1, kBndSphereIn = 2, kBndSphereOut = 4, kBndBox = 8 };
+enum { kRetSuccess = 0, kRetTerminated = 1, kRetMaxIters = 2 };
+
+// ---------------------------------------------------------------------------
+// FP64 helpers: StaticArrays order (cross, left-folded dot)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot3(const double a[3], const double b[3])
+{
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// boris_velocity_update, src/boris/boris.jl:19-31 (three true divisions, as
+// the SVector `2 * t / (1 + t^2)` divides componentwise)
+__device__ __forceinline__ void boris_f64(const double v[3], const double E[3], const double B[3],
+                                          double h, double out[3])
+{
+    double t[3], s[3], vm[3], vmt[3], vp[3], vps[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t[c] = h * B[c];
+    const double den = 1.0 + dot3(t, t);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s[c] = (2.0 * t[c]) / den;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vm[c] = v[c] + h * E[c];
+    cross3(vm, t, vmt);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vmt[c];
+    cross3(vp, s, vps);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = (vm[c] + vps[c]) + h * E[c];
+}
+
+__device__ __forceinline__ double horner_f64(const double* p, int d, double u)
+{
+    double r = p[d];
+    for (int j = d - 1; j >= 0; --j) r = fma(r, u, p[j]);
+    return r;
+}
+
+// update_velocity_multistep, src/boris/multistep_boris.jl:65-142
+__device__ __forceinline__ void multistep_f64(const KParams& p, const double v[3], const double E[3],
+                                              const double B[3], double dt, double out[3])
+{
+    const int n = p.nsub;
+    const double factor = p.q2m * dt / (2.0 * n);
+    double tn[3], en[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tn[c] = factor * B[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) en[c] = factor * E[c];
+    if (p.order_N != 2) {
+        const double tm2 = dot3(tn, tn);
+        double fN, ecf;
+        if (p.order_N == 4) {
+            fN = 1.0 + tm2 / 3.0;
+            ecf = -1.0 / 3.0;
+        } else {
+            fN = (1.0 + tm2 / 3.0) + 2.0 * tm2 * tm2 / 15.0;
+            ecf = -1.0 / 3.0 - 2.0 * tm2 / 15.0;
+        }
+        const double edt = dot3(en, tn);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) en[c] = fN * en[c] + (ecf * edt) * tn[c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tn[c] = fN * tn[c];
+    }
+    const double tau2 = dot3(tn, tn);
+    double c1, c2, c3, c6;
+    if (p.form == 1) {
+        const double rD = 1.0 / horner_f64(p.poly.D, p.poly.deg, tau2);
+        c1 = horner_f64(p.poly.P1, p.poly.deg, tau2) * rD;
+        c2 = horner_f64(p.poly.P2, p.poly.deg - 1, tau2) * rD;
+        c3 = horner_f64(p.poly.P3, p.poly.deg - 1, tau2) * rD;
+        c6 = horner_f64(p.poly.P6, p.poly.deg - 1, tau2) * rD;
+    } else {
+        const double tmag = sqrt(tau2);
+        const double nn = (double)n;
+        if (tmag < 1.0e-4) {
+            c1 = fma(-(2.0 * nn * nn), tau2, 1.0);
+            const double n1 = 2.0 * nn, n3 = 4.0 * nn * nn * nn;
+            c2 = fma(-((n1 + n3) / 3.0), tau2, n1);
+            c3 = fma(-((4.0 * nn * nn + 2.0 * nn * nn * nn * nn) / 3.0), tau2, 2.0 * nn * nn);
+            c6 = (n1 + n3) / 3.0;
+        } else {
+            const double a = atan(tmag);
+            double s, co;
+            sincos(nn * a, &s, &co);
+            const double s2 = 2.0 * s * co;
+            c1 = fma(co, co, -(s * s));
+            c2 = s2 / tmag;
+            c3 = 2.0 * s * s / tau2;
+            c6 = (2.0 * nn - c2) / tau2;
+        }
+    }
+    const double vdt = dot3(v, tn), edt = dot3(en, tn);
+    double vxt[3], ext[3];
+    cross3(v, tn, vxt);
+    cross3(en, tn, ext);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double r = c1 * v[c];
+        r = r + c2 * vxt[c];
+        r = r + (c3 * vdt) * tn[c];
+        r = r + c2 * en[c];
+        r = r + c3 * ext[c];
+        r = r + (c6 * edt) * tn[c];
+        out[c] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 fields
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lerp_f64(double a, double b, double w)
+{
+    return fma(w, b, (1.0 - w) * a);
+}
+
+__device__ __forceinline__ void node3_f64(const DevField& f, long long idx, double o[3])
+{
+    if (f.dtype == 1) {
+        const float* d = (const float*)f.data + 3 * idx;
+        o[0] = (double)d[0]; o[1] = (double)d[1]; o[2] = (double)d[2];
+    } else {
+        const double* d = (const double*)f.data + 3 * idx;
+        o[0] = d[0]; o[1] = d[1]; o[2] = d[2];
+    }
+}
+
+// build_interpolator(CartesianGrid, A::4D, ...) + linear_interp,
+// src/utility/interpolation.jl:119-148, :203-206, :217-235
+__device__ __forceinline__ void grid_f64(const DevField& f, const double x[3], double out[3])
+{
+    int i0[3], i1[3];
+    double w[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int nd = f.n[d];
+        double s = (x[d] - f.origin[d]) * f.inv_h[d];
+        const double smax = (double)(nd - 1);
+        if (s != s || (f.extrap == kExtrapPeriodic && !(fabs(s) < 4.0e18))) {
+            out[0] = out[1] = out[2] = __builtin_nan("");
+            return;
+        }
+        if (f.extrap == kExtrapPeriodic) {
+            const double fl = floor(s);
+            w[d] = s - fl;
+            long long ii = (long long)fl % (long long)nd;
+            if (ii < 0) ii += nd;
+            i0[d] = (int)ii;
+            i1[d] = (i0[d] + 1 == nd) ? 0 : i0[d] + 1;
+        } else {
+            if (f.extrap == kExtrapFillNaN) {
+                if (!(s >= 0.0 && s <= smax)) {
+                    out[0] = out[1] = out[2] = __builtin_nan("");
+                    return;
+                }
+            } else {
+                s = s < 0.0 ? 0.0 : s;
+                s = s > smax ? smax : s;
+            }
+            double fl = floor(s);
+            if (fl > smax - 1.0) fl = smax - 1.0;
+            w[d] = s - fl;
+            i0[d] = (int)fl;
+            i1[d] = i0[d] + 1;
+        }
+    }
+    const long long nx = f.n[0], nxy = (long long)f.n[0] * f.n[1];
+    double a[3], b[3], c00[3], c10[3], c01[3], c11[3];
+    node3_f64(f, i0[0] + nx * i0[1] + nxy * i0[2], a);
+    node3_f64(f, i1[0] + nx * i0[1] + nxy * i0[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c00[c] = lerp_f64(a[c], b[c], w[0]);
+    node3_f64(f, i0[0] + nx * i1[1] + nxy * i0[2], a);
+    node3_f64(f, i1[0] + nx * i1[1] + nxy * i0[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c10[c] = lerp_f64(a[c], b[c], w[0]);
+    node3_f64(f, i0[0] + nx * i0[1] + nxy * i1[2], a);
+    node3_f64(f, i1[0] + nx * i0[1] + nxy * i1[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c01[c] = lerp_f64(a[c], b[c], w[0]);
+    node3_f64(f, i0[0] + nx * i1[1] + nxy * i1[2], a);
+    node3_f64(f, i1[0] + nx * i1[1] + nxy * i1[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c11[c] = lerp_f64(a[c], b[c], w[0]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        out[c] = lerp_f64(lerp_f64(c00[c], c10[c], w[1]), lerp_f64(c01[c], c11[c], w[1]), w[2]);
+}
+
+__device__ __forceinline__ void field_f64(const DevField& f, const double x[3], double t, double o[3])
+{
+    switch (f.kind) {
+    case kFieldUniform:
+        o[0] = f.v[0]; o[1] = f.v[1]; o[2] = f.v[2];
+        return;
+    case kFieldAffine:
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double r = f.v[c] + f.jac[3 * c] * x[0];
+            r = r + f.jac[3 * c + 1] * x[1];
+            r = r + f.jac[3 * c + 2] * x[2];
+            o[c] = r + f.ft[c] * t;
+        }
+        return;
+    case kFieldDipole: {
+        const double r2 = dot3(x, x);
+        const double r = sqrt(r2);
+        const double r5 = (r2 * r2) * r;
+        const double k = f.b0 * ((f.radius * f.radius) * f.radius);
+        const double fac = k / r5;
+        o[0] = fac * ((-3.0 * x[0]) * x[2]);
+        o[1] = fac * ((-3.0 * x[1]) * x[2]);
+        o[2] = fac * (r2 - (3.0 * x[2]) * x[2]);
+        return;
+    }
+    case kFieldGrid:
+        grid_f64(f, x, o);
+        return;
+    default:  // zero
+        o[0] = o[1] = o[2] = 0.0;
+        return;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 helpers (compensated path).  Contractions are written out with fmaf.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dot3f(const float a[3], const float b[3])
+{
+    return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
+}
+__device__ __forceinline__ void cross3f(const float a[3], const float b[3], float o[3])
+{
+    o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
+    o[1] = fmaf(a[2], b[0], -(a[0] * b[2]));
+    o[2] = fmaf(a[0], b[1], -(a[1] * b[0]));
+}
+
+__device__ __forceinline__ void boris_f32(const float v[3], const float E[3], const float B[3],
+                                          float h, float out[3])
+{
+    float t[3], s[3], vm[3], vmt[3], vp[3], vps[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t[c] = h * B[c];
+    const float r = 2.0f / (1.0f + dot3f(t, t));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s[c] = t[c] * r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vm[c] = fmaf(h, E[c], v[c]);
+    cross3f(vm, t, vmt);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vmt[c];
+    cross3f(vp, s, vps);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = fmaf(h, E[c], vm[c] + vps[c]);
+}
+
+__device__ __forceinline__ float horner_f32(const double* p, int d, float u)
+{
+    float r = (float)p[d];
+    for (int j = d - 1; j >= 0; --j) r = fmaf(r, u, (float)p[j]);
+    return r;
+}
+
+__device__ __forceinline__ void multistep_f32(const KParams& p, const float v[3], const float E[3],
+                                              const float B[3], float dt, float out[3])
+{
+    const int n = p.nsub;
+    const float factor = (float)p.q2m * dt / (2.0f * n);
+    float tn[3], en[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tn[c] = factor * B[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) en[c] = factor * E[c];
+    if (p.order_N != 2) {
+        const float tm2 = dot3f(tn, tn);
+        float fN, ecf;
+        if (p.order_N == 4) {
+            fN = fmaf(tm2, 1.0f / 3.0f, 1.0f);
+            ecf = -1.0f / 3.0f;
+        } else {
+            fN = fmaf(tm2, fmaf(tm2, 2.0f / 15.0f, 1.0f / 3.0f), 1.0f);
+            ecf = fmaf(tm2, -2.0f / 15.0f, -1.0f / 3.0f);
+        }
+        const float k = ecf * dot3f(en, tn);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) en[c] = fmaf(fN, en[c], k * tn[c]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tn[c] = fN * tn[c];
+    }
+    const float tau2 = dot3f(tn, tn);
+    float c1, c2, c3, c6;
+    if (p.form == 1) {
+        const float rD = 1.0f / horner_f32(p.poly.D, p.poly.deg, tau2);
+        c1 = horner_f32(p.poly.P1, p.poly.deg, tau2) * rD;
+        c2 = horner_f32(p.poly.P2, p.poly.deg - 1, tau2) * rD;
+        c3 = horner_f32(p.poly.P3, p.poly.deg - 1, tau2) * rD;
+        c6 = horner_f32(p.poly.P6, p.poly.deg - 1, tau2) * rD;
+    } else {
+        const float tmag = sqrtf(tau2);
+        const float nn = (float)n;
+        if (tmag < 1.0e-4f) {
+            c1 = fmaf(-(2.0f * nn * nn), tau2, 1.0f);
+            const float n1 = 2.0f * nn, n3 = 4.0f * nn * nn * nn;
+            c2 = fmaf(-((n1 + n3) / 3.0f), tau2, n1);
+            c3 = fmaf(-((4.0f * nn * nn + 2.0f * nn * nn * nn * nn) / 3.0f), tau2, 2.0f * nn * nn);
+            c6 = (n1 + n3) / 3.0f;
+        } else {
+            const float a = atanf(tmag);
+            float s, co;
+            sincosf(nn * a, &s, &co);
+            c1 = fmaf(co, co, -(s * s));
+            c2 = 2.0f * s * co / tmag;
+            c3 = 2.0f * s * s / tau2;
+            c6 = (2.0f * nn - c2) / tau2;
+        }
+    }
+    const float vdt = dot3f(v, tn), edt = dot3f(en, tn);
+    float vxt[3], ext[3];
+    cross3f(v, tn, vxt);
+    cross3f(en, tn, ext);
+    const float a3 = c3 * vdt, a6 = c6 * edt;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float r = c1 * v[c];
+        r = fmaf(c2, vxt[c], r);
+        r = fmaf(a3, tn[c], r);
+        r = fmaf(c2, en[c], r);
+        r = fmaf(c3, ext[c], r);
+        out[c] = fmaf(a6, tn[c], r);
+    }
+}
+
+// Compensated position: x = frame_origin + wrap * period + hi + lo
+struct Pos32 {
+    float hi[3];
+    float lo[3];
+    int wrap[3];
+};
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
+{
+    s = a + b;
+    const float bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
+
+__device__ __forceinline__ float lerp_f32(float a, float b, float w)
+{
+    return fmaf(w, b, (1.0f - w) * a);
+}
+
+template <typename T>
+__device__ __forceinline__ void node3_f32(const T* __restrict__ d, long long idx, float o[3])
+{
+    const T* q = d + 3 * idx;
+    o[0] = (float)q[0]; o[1] = (float)q[1]; o[2] = (float)q[2];
+}
+
+// Trilinear gather in float from frame-relative coordinates (frame origin =
+// the grid origin, so s = (hi + lo) / h without cancellation).
+template <typename T>
+__device__ __forceinline__ void grid_f32_t(const DevField& f, const Pos32& x, float out[3])
+{
+    int i0[3], i1[3];
+    float w[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int nd = f.n[d];
+        float s = fmaf(x.hi[d], f.inv_hf[d], x.lo[d] * f.inv_hf[d]);
+        const float smax = (float)(nd - 1);
+        if (f.extrap == kExtrapPeriodic) {
+            if (!(fabsf(s) < 1.0e9f)) {
+                out[0] = out[1] = out[2] = __builtin_nanf("");
+                return;
+            }
+            const float fl = floorf(s);
+            w[d] = s - fl;
+            int ii = (int)fl % nd;
+            if (ii < 0) ii += nd;
+            i0[d] = ii;
+            i1[d] = (ii + 1 == nd) ? 0 : ii + 1;
+        } else {
+            if (f.extrap == kExtrapFillNaN) {
+                if (!(s >= 0.0f && s <= smax)) {
+                    out[0] = out[1] = out[2] = __builtin_nanf("");
+                    return;
+                }
+            } else {
+                s = s != s ? 0.0f : s;
+                s = fminf(fmaxf(s, 0.0f), smax);
+            }
+            const float fl = fminf(floorf(s), smax - 1.0f);
+            w[d] = s - fl;
+            i0[d] = (int)fl;
+            i1[d] = i0[d] + 1;
+        }
+    }
+    const T* __restrict__ d = (const T*)f.data;
+    const long long nx = f.n[0], nxy = (long long)f.n[0] * f.n[1];
+    float a[3], b[3], c00[3], c10[3], c01[3], c11[3];
+    node3_f32(d, i0[0] + nx * i0[1] + nxy * i0[2], a);
+    node3_f32(d, i1[0] + nx * i0[1] + nxy * i0[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c00[c] = lerp_f32(a[c], b[c], w[0]);
+    node3_f32(d, i0[0] + nx * i1[1] + nxy * i0[2], a);
+    node3_f32(d, i1[0] + nx * i1[1] + nxy * i0[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c10[c] = lerp_f32(a[c], b[c], w[0]);
+    node3_f32(d, i0[0] + nx * i0[1] + nxy * i1[2], a);
+    node3_f32(d, i1[0] + nx * i0[1] + nxy * i1[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c01[c] = lerp_f32(a[c], b[c], w[0]);
+    node3_f32(d, i0[0] + nx * i1[1] + nxy * i1[2], a);
+    node3_f32(d, i1[0] + nx * i1[1] + nxy * i1[2], b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) c11[c] = lerp_f32(a[c], b[c], w[0]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        out[c] = lerp_f32(lerp_f32(c00[c], c10[c], w[1]), lerp_f32(c01[c], c11[c], w[1]), w[2]);
+}
+
+// absolute position in double (for analytic fields, boundaries and output)
+__device__ __forceinline__ void pos_abs(const KParams& p, const Pos32& x, double o[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        o[d] = (p.frame_origin[d] + (double)x.wrap[d] * p.frame_period[d]) +
+               ((double)x.hi[d] + (double)x.lo[d]);
+}
+
+__device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, const Pos32& x,
+                                          double t, float o[3])
+{
+    switch (f.kind) {
+    case kFieldUniform:
+        o[0] = (float)f.v[0]; o[1] = (float)f.v[1]; o[2] = (float)f.v[2];
+        return;
+    case kFieldGrid:
+        if (f.dtype == 1) grid_f32_t<float>(f, x, o);
+        else grid_f32_t<double>(f, x, o);
+        return;
+    case kFieldAffine: {
+        double xa[3];
+        pos_abs(p, x, xa);
+        float xf[3] = {(float)xa[0], (float)xa[1], (float)xa[2]};
+        const float tf = (float)t;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float r = fmaf((float)f.jac[3 * c], xf[0], (float)f.v[c]);
+            r = fmaf((float)f.jac[3 * c + 1], xf[1], r);
+            r = fmaf((float)f.jac[3 * c + 2], xf[2], r);
+            o[c] = fmaf((float)f.ft[c], tf, r);
+        }
+        return;
+    }
+    case kFieldDipole: {
+        double xa[3];
+        pos_abs(p, x, xa);
+        const float xf[3] = {(float)xa[0], (float)xa[1], (float)xa[2]};
+        // b0 (R/r)^3 / r^2 keeps every intermediate inside float's normal range
+        const float r2 = dot3f(xf, xf);
+        const float ir = rsqrtf(r2);
+        const float q = (float)f.radius * ir;
+        const float fac = (float)f.b0 * (q * q * q) * (ir * ir);
+        o[0] = fac * (-3.0f * xf[0] * xf[2]);
+        o[1] = fac * (-3.0f * xf[1] * xf[2]);
+        o[2] = fac * fmaf(-3.0f * xf[2], xf[2], r2);
+        return;
+    }
+    default:
+        o[0] = o[1] = o[2] = 0.0f;
+        return;
+    }
+}
+
+}  // namespace tpamd

@@ -1,0 +1,231 @@
+# TestParticleAMD.jl -- thin ccall layer from TestParticle.jl to libtpamd.so.
+#
+# Loaded as a package extension of TestParticle.jl, it adds one method of
+# `solve` that dispatches on the device backend `MI355X()`, mirroring the
+# KernelAbstractions entry point
+#     solve(prob, ::Boris{false}, backend::KA.Backend, ensemblealg; dt, ...)
+# (src/boris/boris_kernel.jl:312-382), and extends it to MultistepBoris and the
+# adaptive solvers.  Fields are described with the device menu (DeviceField)
+# rather than Julia closures, since a GPU cannot call host code.
+#
+# Untested here: the build container has no Julia.  The C ABI it calls is
+# include/tpamd.h, which the Python host tests exercise.
+module TestParticleAMD
+
+using TestParticle
+using TestParticle: TraceProblem, Boris, MultistepBoris, build_solution
+using SciMLBase: EnsembleSolution, ReturnCode, LinearInterpolation
+using StaticArrays
+
+const libtpamd = joinpath(@__DIR__, "..", "_build", "libtpamd.so")
+
+struct MI355X
+    device::Int
+end
+MI355X() = MI355X(0)
+
+# mirrors tpamd_solve_desc (include/tpamd.h)
+Base.@kwdef mutable struct SolveDesc
+    alg::Int32 = 0
+    adaptive::Int32 = 0
+    order_N::Int32 = 2
+    nsub::Int32 = 1
+    form::Int32 = 1
+    precision::Int32 = 0
+    save_start::Int32 = 1
+    save_end::Int32 = 1
+    save_everystep::Int32 = 1
+    bnd_flags::Int32 = 0
+    safety::Float64 = 0.0
+    q2m::Float64 = 0.0
+    t0::Float64 = 0.0
+    t1::Float64 = 0.0
+    dt::Float64 = 0.0
+    maxiters::Int64 = 1_000_000
+    savestepinterval::Int64 = 1
+    max_steps::Int64 = 0
+    nout_cap::Int64 = 0
+    r_in::Float64 = 0.0
+    r_out::Float64 = Inf
+    box_lo::NTuple{3, Float64} = (-Inf, -Inf, -Inf)
+    box_hi::NTuple{3, Float64} = (Inf, Inf, Inf)
+    dt_pp::Ptr{Float64} = C_NULL
+    nsteps_pp::Int64 = 0
+    block_size::Int32 = 0
+    reserved::NTuple{7, Int32} = ntuple(_ -> Int32(0), 7)
+end
+
+mutable struct Stats
+    kernel_ms::Float64
+    total_ms::Float64
+    particle_steps::Int64
+    nt::Int64
+    nout::Int64
+    max_len::Int64
+end
+Stats() = Stats(0, 0, 0, 0, 0, 0)
+
+const _ctx = Ref{Ptr{Cvoid}}(C_NULL)
+
+function context(dev::Int)
+    if _ctx[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:tpamd_init, libtpamd), Int32, (Int32, Ptr{Ptr{Cvoid}}), dev, h)
+        rc == 0 || error(unsafe_string(ccall((:tpamd_last_error, libtpamd), Cstring, (Ptr{Cvoid},), C_NULL)))
+        _ctx[] = h[]
+    end
+    return _ctx[]
+end
+
+function check(ctx, rc)
+    rc == 0 && return
+    msg = unsafe_string(ccall((:tpamd_last_error, libtpamd), Cstring, (Ptr{Cvoid},), ctx))
+    rc in (1, 2, 3, 4) && throw(ArgumentError(msg))
+    error(msg)
+end
+
+# --- device field menu -------------------------------------------------------
+abstract type DeviceField end
+struct ZeroDeviceField <: DeviceField end
+struct UniformDeviceField <: DeviceField
+    v::SVector{3, Float64}
+end
+struct DipoleDeviceField <: DeviceField
+    b0::Float64
+    radius::Float64
+end
+"Gridded field: A is Julia's 3 x nx x ny x nz array on uniform ranges x, y, z."
+struct GridDeviceField{T} <: DeviceField
+    A::Array{T, 4}
+    x::AbstractRange
+    y::AbstractRange
+    z::AbstractRange
+    extrap::Int32     # 0 FillExtrap(NaN), 1 ClampExtrap(), 2 periodic (exclusive)
+    storage::Int32    # 0 Float64, 1 Float32
+end
+struct UxBDeviceField <: DeviceField
+    u::SVector{3, Float64}
+    B::DeviceField
+end
+
+handle(ctx, ::ZeroDeviceField, cache) = _mk(ctx) do h
+    ccall((:tpamd_field_zero, libtpamd), Int32, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}), ctx, h)
+end
+handle(ctx, f::UniformDeviceField, cache) = _mk(ctx) do h
+    v = Vector(f.v)
+    ccall((:tpamd_field_uniform, libtpamd), Int32, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Ptr{Cvoid}}), ctx, v, h)
+end
+handle(ctx, f::DipoleDeviceField, cache) = _mk(ctx) do h
+    ccall((:tpamd_field_dipole, libtpamd), Int32, (Ptr{Cvoid}, Float64, Float64, Ptr{Ptr{Cvoid}}),
+          ctx, f.b0, f.radius, h)
+end
+function handle(ctx, f::GridDeviceField{T}, cache) where {T}
+    get!(cache, f) do
+        _mk(ctx) do h
+            origin = [first(f.x), first(f.y), first(f.z)]
+            spacing = [step(f.x), step(f.y), step(f.z)]
+            GC.@preserve f ccall((:tpamd_field_grid, libtpamd), Int32,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int64, Int64, Int64, Ptr{Float64}, Ptr{Float64},
+                 Int32, Int32, Ptr{Ptr{Cvoid}}),
+                ctx, f.A, T === Float64 ? 0 : 1, size(f.A, 2), size(f.A, 3), size(f.A, 4),
+                origin, spacing, f.extrap, f.storage, h)
+        end
+    end
+end
+function handle(ctx, f::UxBDeviceField, cache)
+    hB = handle(ctx, f.B, cache)
+    _mk(ctx) do h
+        u = Vector(f.u)
+        ccall((:tpamd_field_uxb, libtpamd), Int32, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}),
+              ctx, u, hB, h)
+    end
+end
+function _mk(f, ctx)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ctx, f(h))
+    return h[]
+end
+
+# --- solve --------------------------------------------------------------------
+"""
+    solve(prob::TraceProblem, alg, backend::MI355X, [ensemblealg]; dt, trajectories=1,
+          savestepinterval=1, isoutside=nothing, save_start=true, save_end=true,
+          save_everystep=true, maxiters=1_000_000, precision=:fp64)
+
+Device counterpart of `solve(prob, Boris(), backend; ...)`
+(src/boris/boris_kernel.jl:312-382) for Boris, MultistepBoris and the adaptive
+variants.  `prob.p = (q2m, m, E::DeviceField, B::DeviceField, F)`.
+"""
+function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensemblealg = nothing;
+        dt = 0.0, trajectories::Int = 1, savestepinterval::Int = 1, isoutside = nothing,
+        save_start::Bool = true, save_end::Bool = true, save_everystep::Bool = true,
+        maxiters::Int = 1_000_000, precision::Symbol = :fp64, nout_cap::Int = 0)
+    ctx = context(backend.device)
+    q2m, _, E, B, _ = prob.p
+    T = Float64
+    # initial states: prob_func on the host, as the reference does (boris.jl:501-503)
+    xv0 = Matrix{Float64}(undef, 6, trajectories)
+    for i in 1:trajectories
+        xv0[:, i] .= prob.prob_func(prob, (sim_id = i, repeat = false)).u0
+    end
+    d = SolveDesc(q2m = q2m, t0 = prob.tspan[1], t1 = prob.tspan[2], dt = dt,
+                  savestepinterval = savestepinterval, maxiters = maxiters,
+                  save_start = save_start, save_end = save_end, save_everystep = save_everystep,
+                  precision = precision === :fp32c ? 1 : 0)
+    if alg isa MultistepBoris
+        d.alg = 1; d.order_N = typeof(alg).parameters[1]; d.nsub = alg.n
+    end
+    d.adaptive = alg.safety > 0 ? 1 : 0
+    d.safety = alg.safety
+    isoutside === nothing || apply_boundary!(d, isoutside)
+    nt, nout = Ref{Int64}(0), Ref{Int64}(0)
+    if d.adaptive == 0
+        check(C_NULL, ccall((:tpamd_plan, libtpamd), Int32,
+            (Ptr{Cvoid}, Ref{SolveDesc}, Ref{Int64}, Ref{Int64}), C_NULL, d, nt, nout))
+        d.nout_cap = nout[]
+    else
+        d.nout_cap = nout_cap > 0 ? nout_cap : (save_everystep ? 1024 : 2)
+    end
+    cache = IdDict()
+    hE, hB = handle(ctx, E, cache), handle(ctx, B, cache)
+    cap = d.nout_cap
+    u = Array{Float64}(undef, 6, cap, trajectories)
+    t = Array{Float64}(undef, cap, trajectories)
+    len = Vector{Int32}(undef, trajectories)
+    ret = Vector{Int32}(undef, trajectories)
+    st = Ref(Stats())
+    GC.@preserve xv0 u t len ret begin
+        rc = ccall((:tpamd_solve, libtpamd), Int32,
+            (Ptr{Cvoid}, Ref{SolveDesc}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Int64,
+             Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ref{Stats}),
+            ctx, d, hE, hB, xv0, trajectories, u, t, len, ret, st)
+        check(ctx, rc)
+    end
+    sols = map(1:trajectories) do i
+        n = len[i]
+        ts = t[1:n, i]
+        us = [SVector{6, T}(u[:, k, i]) for k in 1:n]
+        rcode = ret[i] == 0 ? ReturnCode.Success : ReturnCode.Terminated
+        build_solution(prob, :boris, ts, us; interp = LinearInterpolation(ts, us), retcode = rcode)
+    end
+    return EnsembleSolution(sols, st[].kernel_ms * 1e-3, true)
+end
+
+# isoutside menu: (:nan, :sphere_in => R, :sphere_out => R, :box => (lo, hi))
+function apply_boundary!(d::SolveDesc, spec)
+    for s in (spec isa Tuple ? spec : (spec,))
+        if s === :nan
+            d.bnd_flags |= 1
+        elseif s isa Pair && s.first === :sphere_in
+            d.bnd_flags |= 2; d.r_in = s.second
+        elseif s isa Pair && s.first === :sphere_out
+            d.bnd_flags |= 4; d.r_out = s.second
+        elseif s isa Pair && s.first === :box
+            d.bnd_flags |= 8; d.box_lo = Tuple(s.second[1]); d.box_hi = Tuple(s.second[2])
+        else
+            throw(ArgumentError("isoutside must come from the device menu"))
+        end
+    end
+end
+
+end # module

@@ -1,0 +1,66 @@
+"""Device-resident ensembles: initial states and records live in HBM (torch
+tensors used as plain device buffers), and each run is one call of
+tpamd_solve_device on the caller's stream.  For repeated solves (benchmarks,
+optimisation loops) where host<->device copies are not part of the work."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .solve import _fields, make_desc, plan
+
+
+class DeviceEnsemble:
+    def __init__(self, prob, alg, xv0: np.ndarray, *, device: int = 0, nout_cap=None,
+                 dts=None, **desc_kw):
+        import torch
+
+        self.torch = torch
+        self.ctx = _lib.context(device)
+        self.dev = torch.device(f"cuda:{device}")
+        self.E, self.B = _fields(prob)
+        self.N = int(xv0.shape[0])
+        self.desc = make_desc(prob, alg, dts=dts, **desc_kw)
+        if alg.adaptive:
+            cap = int(nout_cap if nout_cap is not None else 2)
+        else:
+            if dts is not None:
+                self.desc.dt_pp = None
+            cap = plan(self.desc)[1]
+        self.desc.nout_cap = cap
+        self.cap = cap
+        self.xv0 = torch.from_numpy(np.ascontiguousarray(xv0, dtype=np.float64)).to(self.dev)
+        self.dts = None
+        if dts is not None:
+            self.dts = torch.from_numpy(np.ascontiguousarray(dts, dtype=np.float64)).to(self.dev)
+            self.desc.dt_pp = self.dts.data_ptr()
+        self.u = torch.empty((self.N, max(cap, 1), 6), dtype=torch.float64, device=self.dev)
+        self.t = torch.empty((self.N, max(cap, 1)), dtype=torch.float64, device=self.dev)
+        self.len = torch.empty(self.N, dtype=torch.int32, device=self.dev)
+        self.ret = torch.empty(self.N, dtype=torch.int32, device=self.dev)
+        self.hE, self.hB = self.E.handle(self.ctx), self.B.handle(self.ctx)
+        torch.cuda.synchronize(self.dev)
+
+    def run(self) -> _lib.Stats:
+        """One solve on torch's current stream; returns the library's stats
+        (kernel_ms from HIP events on that stream)."""
+        st = _lib.Stats()
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        with self.ctx.lock:
+            rc = self.ctx.L.tpamd_solve_device(
+                self.ctx.h, C.byref(self.desc), self.hE, self.hB, self.xv0.data_ptr(), self.N,
+                self.u.data_ptr(), self.t.data_ptr(), self.len.data_ptr(), self.ret.data_ptr(),
+                C.c_void_p(stream), C.byref(st))
+        self.ctx.check(rc, st)
+        return st
+
+    def final_states(self) -> np.ndarray:
+        idx = (self.len.long() - 1).clamp(min=0)
+        ar = self.torch.arange(self.N, device=self.dev)
+        return self.u[ar, idx].cpu().numpy()
+
+    def host(self):
+        return (self.u.cpu().numpy(), self.t.cpu().numpy(), self.len.cpu().numpy(),
+                self.ret.cpu().numpy())
