@@ -123,8 +123,12 @@ typedef struct tpamd_solve_desc {
     const double* dt_pp;      /* optional per-particle dt (fixed step): tspan_i = (t0, t0 + nsteps_pp dt_i) */
     int64_t nsteps_pp;
     int32_t block_size;       /* 0 = library default */
-    int32_t reserved[7];
+    int32_t flags;            /* TPAMD_FLAG_* bits */
+    int32_t reserved[6];
 } tpamd_solve_desc;
+
+/* descriptor flags */
+#define TPAMD_FLAG_GENERIC_KERNELS 1  /* skip the gridded-field specialisations (for cross-checks) */
 
 typedef struct tpamd_stats {
     double kernel_ms;         /* device time of the push kernel, HIP events on the solve stream */

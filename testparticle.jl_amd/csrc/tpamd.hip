@@ -6,6 +6,7 @@
 // step, synchronises, and copies the full 6 x N state to the host for every
 // saved record (src/boris/boris_kernel.jl:130-236).
 #include "tpamd_device.h"
+#include "tpamd_grid.h"
 #include "tpamd.h"
 
 #include <hip/hip_runtime.h>
@@ -393,6 +394,82 @@ __global__ void __launch_bounds__(256) k_adaptive(const KParams p)
     add_steps(p, steps);
 }
 
+// ---------------------------------------------------------------------------
+// Specialised gridded-B policies (see tpamd_grid.h).  NS = 0 is the Boris
+// update, NS >= 1 the rational multistep update with n = NS subcycles.
+// ---------------------------------------------------------------------------
+template <int EMODE, int NS, bool PER, typename T>
+struct S64G : S64 {
+    __device__ void bfield(const KParams& p, double B[3]) const
+    {
+        int i0[3], i1[3];
+        double w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double s = (pos.x[d] - p.B.origin[d]) * p.B.inv_h[d];
+            if (!locate_f64<PER>(p.B, d, s, i0[d], i1[d], w[d])) {
+                B[0] = B[1] = B[2] = __builtin_nan("");
+                return;
+            }
+        }
+        unsigned ox[2], oy[2], oz[2];
+        offsets(p.B, i0, i1, sizeof(T), ox, oy, oz);
+        trilinear<T, double>(p.B.data, ox, oy, oz, w, B);
+    }
+    template <int ALG>
+    __device__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
+    {
+        double E[3], B[3];
+        bfield(p, B);
+        if (EMODE == kEUxB) cross3(B, p.E.v, E);
+        else { E[0] = p.E.v[0]; E[1] = p.E.v[1]; E[2] = p.E.v[2]; }
+        if (NS == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
+        else multistep_f64_n<NS == 0 ? 1 : NS>(p, vin, E, B, h, out);
+    }
+    __device__ double bmag(const KParams& p, double) const
+    {
+        double B[3];
+        bfield(p, B);
+        return sqrt(dot3(B, B));
+    }
+};
+
+template <int EMODE, int NS, bool PER>
+struct S32G : S32C {
+    __device__ void bfield(const KParams& p, float B[3]) const
+    {
+        int i0[3], i1[3];
+        float w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float s = fmaf(pos.hi[d], p.B.inv_hf[d], pos.lo[d] * p.B.inv_hf[d]);
+            if (!locate_f32<PER>(p.B, d, s, i0[d], i1[d], w[d])) {
+                B[0] = B[1] = B[2] = __builtin_nanf("");
+                return;
+            }
+        }
+        unsigned ox[2], oy[2], oz[2];
+        offsets(p.B, i0, i1, 4u, ox, oy, oz);
+        trilinear<float, float>(p.B.data, ox, oy, oz, w, B);
+    }
+    template <int ALG>
+    __device__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
+    {
+        float E[3], B[3];
+        bfield(p, B);
+        if (EMODE == kEUxB) cross3f(B, p.f32.u, E);
+        else { E[0] = p.f32.e0[0]; E[1] = p.f32.e0[1]; E[2] = p.f32.e0[2]; }
+        if (NS == 0) boris_f32_fast(vin, E, B, (float)((p.q2m * 0.5) * h), out);
+        else multistep_f32_n<NS == 0 ? 1 : NS>(p, vin, E, B, (float)(p.q2m_2n * h), out);
+    }
+    __device__ double bmag(const KParams& p, double) const
+    {
+        float B[3];
+        bfield(p, B);
+        return (double)sqrtf(dot3f(B, B));
+    }
+};
+
 __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const double* __restrict__ x,
                                                     long long n, double t, int which, int prec,
                                                     double* __restrict__ out)
@@ -596,6 +673,83 @@ void launch_solve(const KParams& p, int adaptive, int alg, dim3 grid, dim3 block
     }
 }
 
+// one specialised policy: fixed step for every NS, adaptive for Boris (NS = 0)
+template <class S, int NS>
+void launch_spec(const KParams& p, int adaptive, dim3 g, dim3 b, hipStream_t st)
+{
+    if constexpr (NS == 0) {
+        if (adaptive) {
+            hipLaunchKernelGGL((k_adaptive<S, 0>), g, b, 0, st, p);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_fixed<S, NS == 0 ? 0 : 1>), g, b, 0, st, p);
+}
+
+template <int EMODE, int NS>
+void launch_g32(const KParams& p, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
+{
+    if (per) launch_spec<S32G<EMODE, NS, true>, NS>(p, adaptive, g, b, st);
+    else launch_spec<S32G<EMODE, NS, false>, NS>(p, adaptive, g, b, st);
+}
+
+template <int EMODE, int NS, typename T>
+void launch_g64(const KParams& p, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
+{
+    if (per) launch_spec<S64G<EMODE, NS, true, T>, NS>(p, adaptive, g, b, st);
+    else launch_spec<S64G<EMODE, NS, false, T>, NS>(p, adaptive, g, b, st);
+}
+
+template <int EMODE, typename T>
+void launch_g64_ns(const KParams& p, int ns, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
+{
+    if (ns == 0) launch_g64<EMODE, 0, T>(p, per, adaptive, g, b, st);
+    else if (ns == 1) launch_g64<EMODE, 1, T>(p, per, adaptive, g, b, st);
+    else launch_g64<EMODE, 2, T>(p, per, adaptive, g, b, st);
+}
+
+template <int EMODE>
+void launch_g32_ns(const KParams& p, int ns, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
+{
+    if (ns == 0) launch_g32<EMODE, 0>(p, per, adaptive, g, b, st);
+    else if (ns == 1) launch_g32<EMODE, 1>(p, per, adaptive, g, b, st);
+    else launch_g32<EMODE, 2>(p, per, adaptive, g, b, st);
+}
+
+// Launches a specialised kernel when the solve is a gridded-B one it covers;
+// returns false (nothing launched) otherwise.
+bool launch_specialised(const KParams& p, const tpamd_solve_desc* d, const tpamd_field* E,
+                        const tpamd_field* B, dim3 g, dim3 b, hipStream_t st)
+{
+    if (d->flags & TPAMD_FLAG_GENERIC_KERNELS) return false;
+    if (B->d.kind != kFieldGrid) return false;
+    int emode;
+    if (E->d.kind == kFieldUxB) emode = kEUxB;        // setup_fields checked E->src == B
+    else if (E->d.kind == kFieldZero || E->d.kind == kFieldUniform) emode = kEUniform;
+    else return false;
+    int ns;
+    if (d->alg == TPAMD_ALG_BORIS) ns = 0;
+    else if (d->form == TPAMD_FORM_RATIONAL && (d->nsub == 1 || d->nsub == 2)) ns = d->nsub;
+    else return false;
+    if (d->adaptive && ns != 0) return false;
+    const double es = B->d.dtype == 1 ? 4.0 : 8.0;
+    if (3.0 * es * (double)B->d.n[0] * (double)B->d.n[1] * (double)B->d.n[2] >= 4294967296.0)
+        return false;                                  // 32-bit byte offsets
+    const bool per = B->d.extrap == kExtrapPeriodic;
+    if (d->precision == TPAMD_FP32C) {
+        if (B->d.dtype != 1) return false;
+        if (emode == kEUxB) launch_g32_ns<kEUxB>(p, ns, per, d->adaptive, g, b, st);
+        else launch_g32_ns<kEUniform>(p, ns, per, d->adaptive, g, b, st);
+    } else if (B->d.dtype == 1) {
+        if (emode == kEUxB) launch_g64_ns<kEUxB, float>(p, ns, per, d->adaptive, g, b, st);
+        else launch_g64_ns<kEUniform, float>(p, ns, per, d->adaptive, g, b, st);
+    } else {
+        if (emode == kEUxB) launch_g64_ns<kEUxB, double>(p, ns, per, d->adaptive, g, b, st);
+        else launch_g64_ns<kEUniform, double>(p, ns, per, d->adaptive, g, b, st);
+    }
+    return true;
+}
+
 int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* E,
                const tpamd_field* B, const double* d_xv0, int64_t N, double* d_out_u,
                double* d_out_t, int32_t* d_len, int32_t* d_ret, hipStream_t st,
@@ -613,11 +767,26 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     std::memset(&p, 0, sizeof p);
     rc = setup_fields(ctx, E, B, desc->precision, p);
     if (rc) return rc;
-    if (desc->alg == TPAMD_ALG_MULTISTEP && desc->form == TPAMD_FORM_RATIONAL)
+    if (desc->alg == TPAMD_ALG_MULTISTEP && desc->form == TPAMD_FORM_RATIONAL) {
         fill_poly(desc->nsub, p.poly);
+        for (int j = 0; j <= kRationalNMax; ++j) {
+            p.f32.D[j] = (float)p.poly.D[j];
+            p.f32.P1[j] = (float)p.poly.P1[j];
+        }
+        for (int j = 0; j < kRationalNMax; ++j) {
+            p.f32.P2[j] = (float)p.poly.P2[j];
+            p.f32.P3[j] = (float)p.poly.P3[j];
+            p.f32.P6[j] = (float)p.poly.P6[j];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        p.f32.u[c] = (float)E->d.v[c];     // UxB drift (kind UxB) or uniform E value
+        p.f32.e0[c] = (float)E->d.v[c];
+    }
     p.alg = desc->alg;
     p.order_N = desc->order_N;
     p.nsub = desc->alg == TPAMD_ALG_MULTISTEP ? desc->nsub : 1;
+    p.q2m_2n = desc->q2m / (2.0 * p.nsub);
     p.form = desc->form;
     p.save_start = desc->save_start != 0;
     p.save_end = desc->save_end != 0;
@@ -659,8 +828,10 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     if (N > 0) {
         const dim3 block(bs), grid((unsigned)((N + bs - 1) / bs));
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-        if (desc->precision == TPAMD_FP64) launch_solve<S64>(p, desc->adaptive, p.alg, grid, block, st);
-        else launch_solve<S32C>(p, desc->adaptive, p.alg, grid, block, st);
+        if (!launch_specialised(p, desc, E, B, grid, block, st)) {
+            if (desc->precision == TPAMD_FP64) launch_solve<S64>(p, desc->adaptive, p.alg, grid, block, st);
+            else launch_solve<S32C>(p, desc->adaptive, p.alg, grid, block, st);
+        }
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     }

@@ -55,9 +55,22 @@ struct Poly {
     int pad;
 };
 
+// float copies of per-solve constants for the FP32C specialised policies
+struct F32Consts {
+    float D[kRationalNMax + 1];
+    float P1[kRationalNMax + 1];
+    float P2[kRationalNMax];
+    float P3[kRationalNMax];
+    float P6[kRationalNMax];
+    float u[3];    // UxB drift velocity
+    float e0[3];   // uniform E
+};
+
 struct KParams {
     DevField E, B;
     Poly poly;
+    F32Consts f32;
+    double q2m_2n;            // q/m / (2 n), multistep
     int alg, order_N, nsub, form;
     int save_start, save_end, save_every, bnd_flags;
     int e_is_uxb, frame_periodic, pad0, pad1;

@@ -52,7 +52,8 @@ Base.@kwdef mutable struct SolveDesc
     dt_pp::Ptr{Float64} = C_NULL
     nsteps_pp::Int64 = 0
     block_size::Int32 = 0
-    reserved::NTuple{7, Int32} = ntuple(_ -> Int32(0), 7)
+    flags::Int32 = 0
+    reserved::NTuple{6, Int32} = ntuple(_ -> Int32(0), 6)
 end
 
 mutable struct Stats

@@ -26,6 +26,7 @@ FORM_REFERENCE, FORM_RATIONAL = 0, 1
 FP64, FP32C = 0, 1
 BND_NAN, BND_SPHERE_IN, BND_SPHERE_OUT, BND_BOX = 1, 2, 4, 8
 RET_SUCCESS, RET_TERMINATED, RET_MAXITERS = 0, 1, 2
+FLAG_GENERIC_KERNELS = 1
 RETCODE_NAMES = {RET_SUCCESS: "Success", RET_TERMINATED: "Terminated", RET_MAXITERS: "MaxIters"}
 
 
@@ -54,7 +55,7 @@ class SolveDesc(C.Structure):
         ("nout_cap", C.c_int64), ("r_in", C.c_double), ("r_out", C.c_double),
         ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
         ("dt_pp", C.c_void_p), ("nsteps_pp", C.c_int64), ("block_size", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("flags", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 

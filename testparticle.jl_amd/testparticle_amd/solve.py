@@ -66,7 +66,8 @@ def initial_states(prob: TraceProblem, trajectories: int, seed=None) -> np.ndarr
 def make_desc(prob: TraceProblem, alg: AbstractBoris, *, dt=None, savestepinterval=1,
               isoutside=None, save_start=True, save_end=True, save_everystep=True,
               maxiters=1_000_000, precision="fp64", form="rational", max_steps=0,
-              nout_cap=0, dts=None, nsteps=0, workgroup_size=0) -> _lib.SolveDesc:
+              nout_cap=0, dts=None, nsteps=0, workgroup_size=0,
+              kernels="auto") -> _lib.SolveDesc:
     q2m = float(prob.p[0])
     d = _lib.SolveDesc()
     if isinstance(alg, MultistepBoris):
@@ -98,6 +99,9 @@ def make_desc(prob: TraceProblem, alg: AbstractBoris, *, dt=None, savestepinterv
     d.max_steps = int(max_steps)
     d.nout_cap = int(nout_cap)
     d.block_size = int(workgroup_size or 0)
+    if kernels not in ("auto", "generic"):
+        raise _lib.ArgumentError("kernels must be 'auto' or 'generic'")
+    d.flags = _lib.FLAG_GENERIC_KERNELS if kernels == "generic" else 0
     b = isoutside.condition if hasattr(isoutside, "condition") else isoutside
     if b is None:
         b = Boundary()
@@ -148,13 +152,14 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
           save_work: bool = False, maxiters: int = 1_000_000, batch_size=None, seed=None,
           workgroup_size: int = 0, precision: str = "fp64", form: str = "rational",
           max_steps: int = 0, nout_cap=None, dts=None, nsteps: int = 0,
-          device=None) -> EnsembleSolution:
+          kernels: str = "auto", device=None) -> EnsembleSolution:
     """Trace an ensemble with the Boris family on the GPU.
 
     Extensions beyond the reference's keywords: `precision` ("fp64" |
     "fp32c"), `form` ("rational" | "reference", multistep coefficients),
     `max_steps` (adaptive step cap), `dts` + `nsteps` (per-particle step,
-    fixed-step only), `nout_cap` (adaptive record capacity)."""
+    fixed-step only), `nout_cap` (adaptive record capacity), `kernels`
+    ("auto" | "generic": skip the gridded-field specialisations)."""
     backend, _ens = _split_positional(args)
     if save_fields or save_work:
         raise NotImplementedError("save_fields / save_work are not provided by the device path")
@@ -170,7 +175,8 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
     desc = make_desc(prob, alg, dt=dt, savestepinterval=savestepinterval, isoutside=isoutside,
                      save_start=save_start, save_end=save_end, save_everystep=save_everystep,
                      maxiters=maxiters, precision=precision, form=form, max_steps=max_steps,
-                     dts=dts_arr, nsteps=nsteps, workgroup_size=workgroup_size)
+                     dts=dts_arr, nsteps=nsteps, workgroup_size=workgroup_size,
+                     kernels=kernels)
     if dts_arr is not None:
         desc.dt_pp = dts_arr.ctypes.data
     if not alg.adaptive:

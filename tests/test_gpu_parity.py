@@ -271,14 +271,43 @@ def turb64():
     return w, A, ax, B, E
 
 
-@pytest.mark.parametrize("N,n", [(4, 2), (2, 1), (6, 3)])
-def test_parity_turbulent_multistep(turb64, N, n):
+@pytest.mark.parametrize("kernels", ["auto", "generic"])
+@pytest.mark.parametrize("N,n", [(4, 2), (2, 1), (6, 3), (2, 0)])
+def test_parity_turbulent_multistep(turb64, N, n, kernels):
+    # n = 0 stands for the standard Boris update; "auto" takes the gridded-B
+    # specialisations where they apply (n <= 2), "generic" never does
     w, A, ax, B, E = turb64
     prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, E, B, TP.ZeroField()))
-    alg = TP.MultistepBoris(N, n=n)
-    g = TP.solve(prob, alg, dt=w.dt, savestepinterval=100)
+    alg = TP.Boris() if n == 0 else TP.MultistepBoris(N, n=n)
+    g = TP.solve(prob, alg, dt=w.dt, savestepinterval=100, kernels=kernels)
     o = run_oracle(prob, alg, dt=w.dt, savestepinterval=100)
     assert_bitwise(g, o)
+
+
+@pytest.mark.parametrize("extrap", ["periodic", "clamp", "fill_nan"])
+def test_parity_specialised_f32_storage_and_extrap(turb64, extrap):
+    # FP64 arithmetic on float storage, every wrap mode, uniform E
+    w, A, ax, _, _ = turb64
+    bc = {"periodic": TP.PeriodicBC(), "clamp": TP.ClampExtrap(), "fill_nan": TP.FillExtrap()}[extrap]
+    B = TP.GridField(A.astype(np.float32), *ax, bc=bc)
+    prob = TP.TraceProblem(w.xv0[:512], (0.0, 300 * w.dt),
+                           (w.q2m, TP.mi, TP.UniformField([1e-4, -2e-4, 0.0]), B, TP.ZeroField()))
+    for alg in (TP.Boris(), TP.MultistepBoris(4, n=2)):
+        g = TP.solve(prob, alg, dt=w.dt, savestepinterval=30)
+        o = run_oracle(prob, alg, dt=w.dt, savestepinterval=30)
+        assert_bitwise(g, o)
+
+
+def test_fp32c_specialised_matches_generic(turb64):
+    w, A, ax, _, _ = turb64
+    B = TP.GridField(A.astype(np.float32), *ax, bc=TP.PeriodicBC())
+    E = TP.UxBField(w.extra["u"], B)
+    prob = TP.TraceProblem(w.xv0, (0.0, 10 * w.dt), (w.q2m, TP.mi, E, B, TP.ZeroField()))
+    for alg in (TP.Boris(), TP.MultistepBoris(4, n=2)):
+        a = TP.solve(prob, alg, dt=w.dt, precision="fp32c").final_states()
+        b = TP.solve(prob, alg, dt=w.dt, precision="fp32c", kernels="generic").final_states()
+        d = np.linalg.norm(a[:, :3] - b[:, :3], axis=1) / np.linalg.norm(b[:, :3] - w.xv0[:, :3], axis=1)
+        assert np.median(d) < 1e-5, np.median(d)
 
 
 def test_parity_turbulent_stored_E_f32_storage(turb64):
