@@ -99,7 +99,7 @@ def bench_turbulent(args):
     res = {}
     for prec, storage in (("fp32c", "f32"), ("fp64", "f64")):
         A = A64 if storage == "f64" else A64.astype(np.float32)
-        B = TP.GridField(A, *ax, bc=TP.PeriodicBC())
+        B = TP.GridField(A, *ax, bc=TP.PeriodicBC(), layout=args.layout)
         E = TP.UxBField(w.extra["u"], B)
         prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, E, B, TP.ZeroField()))
         log(f"[bench] {prec}: uploading and warming up")
@@ -136,7 +136,7 @@ def bench_turbulent(args):
         "data": "synthetic (seeded k^-5/3 turbulent field and particle states)",
         "config": {"workload": f"turbulent_box_{n}^3_multistep4_n2", "particles": N,
                    "grid": n, "steps_per_solve": nsteps, "alg": "MultistepBoris4(n=2)",
-                   "E": "-u x B in-kernel", "saves": "start+end",
+                   "E": "-u x B in-kernel", "saves": "start+end", "field_layout": args.layout,
                    "parallelism": "single GPU, one thread per particle"},
         "fp64_value": ps / res["fp64"]["wall"],
         "fp32c_over_fp64": res["fp64"]["wall"] / r32["wall"],
@@ -147,13 +147,20 @@ def bench_turbulent(args):
                               "median_rel_position": float(np.median(err_pos))},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_fixed<S32C, multistep>", "kernel_ms": r32["kernel_ms"],
+                     "kernel": "k_fixed<S32G<UxB, n=2, periodic, cells>>", "kernel_ms": r32["kernel_ms"],
                      "algorithmic_bytes_per_particle_step": GATHER_BYTES["f32"]},
     }
+    # HBM traffic per launch: FETCH_SIZE + WRITE_SIZE per particle-step from the
+    # rocprofv3 --pmc passes recorded under profiles/ (scaled by the copy-probe
+    # calibration), times this launch's particle-steps
     prof = os.path.join(ROOT, "profiles", f"traffic_turbulent_box_{n}.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            out["roofline"]["traffic"] = json.load(f).get("bytes_per_launch")
+            t = json.load(f)
+        if t.get("layout") == args.layout:
+            out["roofline"]["traffic"] = ps * (t["fetch_bytes_per_particle_step"] +
+                                               t["write_bytes_per_particle_step"])
+            out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
     if args.cpu_baseline:
         log("[bench] CPU baseline (oracle, all host cores)")
         out["cpu_baseline"] = cpu_baseline(w, A64, ax, cpu_threads())
@@ -162,20 +169,41 @@ def bench_turbulent(args):
     return out
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size: under rocprofv3 --pmc it gives the
+    counter reading (FETCH_SIZE / WRITE_SIZE) per byte actually moved, which
+    scales the pusher's counters to HBM bytes."""
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=1 << 22)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--nsteps", type=int, default=5000)
+    ap.add_argument("--layout", choices=("cells", "nodes"), default="cells",
+                    help="grid storage on the device (cells: one cache line per gather)")
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the ensemble pusher runs on a single GPU "
               "(--gpus 1); multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     out = bench_turbulent(args)
     print(json.dumps(out), flush=True)
 

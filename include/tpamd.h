@@ -83,7 +83,8 @@ typedef enum tpamd_boundary {
     TPAMD_BND_NAN = 1,        /* any(isnan, u) */
     TPAMD_BND_SPHERE_IN = 2,  /* norm(x) < r_in  (e.g. r < R_E) */
     TPAMD_BND_SPHERE_OUT = 4, /* norm(x) > r_out */
-    TPAMD_BND_BOX = 8         /* x outside [box_lo, box_hi] in any component */
+    TPAMD_BND_BOX = 8,        /* x outside [box_lo, box_hi] in any component */
+    TPAMD_BND_GRID = 16       /* x outside the node span of B's grid (open boundary) */
 } tpamd_boundary;
 
 /* Per-trajectory return codes (SciMLBase ReturnCode) */
@@ -165,6 +166,13 @@ int32_t tpamd_field_grid(tpamd_ctx* ctx, const void* data, int32_t dtype, int64_
                          int64_t ny, int64_t nz, const double origin[3],
                          const double spacing[3], int32_t extrap, int32_t storage_dtype,
                          tpamd_field** out);
+/* Builds (enable = 1) or drops (0) a cell-replicated copy of a gridded field:
+ * per cell the 8 corner nodes x 3 components in one block (float: 128 B, one
+ * cache line; double: 192 B), so a trilinear gather reads one contiguous
+ * block instead of eight scattered nodes.  Costs about 10x (float) / 8x
+ * (double) the node array in device memory; the cell array must stay below
+ * 4 GiB.  Results are unchanged (same values, same blend order). */
+int32_t tpamd_field_grid_cells(tpamd_ctx* ctx, tpamd_field* f, int32_t enable);
 /* E = -u x B(x) for a uniform drift u; `B` must be the B field of the solve. */
 int32_t tpamd_field_uxb(tpamd_ctx* ctx, const double u[3], const tpamd_field* B,
                         tpamd_field** out);
@@ -199,6 +207,37 @@ int32_t tpamd_solve_device(tpamd_ctx* ctx, const tpamd_solve_desc* desc,
                            const tpamd_field* E, const tpamd_field* B, const double* d_xv0,
                            int64_t N, double* d_out_u, double* d_out_t, int32_t* d_out_len,
                            int32_t* d_retcode, void* stream, tpamd_stats* stats);
+
+/* ---- batched ensembles ---------------------------------------------------- *
+ * Many field instances with particle populations of uneven size and lifetime
+ * in one persistent launch.  Instance k owns particles [offsets[k],
+ * offsets[k+1]) of the concatenated state array; one descriptor applies to
+ * all instances.  The reference runs such work as one solve per field
+ * (src/boris/boris.jl:88-215 / adaptive_boris.jl:21-71 per instance); this
+ * entry point has no single reference counterpart. */
+typedef struct tpamd_batch tpamd_batch;
+
+/* Binds device buffers (as tpamd_solve_device), uploads the per-instance
+ * parameter table and the work chunks, and captures the launch in a HIP graph.
+ * E[k], B[k] stay owned by the caller and must outlive the batch. */
+int32_t tpamd_batch_create(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t n_inst,
+                           const tpamd_field* const* E, const tpamd_field* const* B,
+                           const int64_t* offsets, const double* d_xv0, double* d_out_u,
+                           double* d_out_t, int32_t* d_out_len, int32_t* d_retcode,
+                           tpamd_batch** out);
+/* Times every (block size, particles per thread) pair on `stream`, keeps the
+ * fastest and re-captures the graph.  Returns the chosen pair. */
+int32_t tpamd_batch_autotune(tpamd_batch* b, void* stream, int32_t* block_size, int32_t* ppt);
+/* One replay of the captured launch on `stream` (NULL = the context's); blocks
+ * until it has completed. */
+int32_t tpamd_batch_run(tpamd_batch* b, void* stream, tpamd_stats* stats);
+int32_t tpamd_batch_free(tpamd_batch* b);
+/* Host-buffer convenience: create, run once, copy back, free. */
+int32_t tpamd_solve_batch(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t n_inst,
+                          const tpamd_field* const* E, const tpamd_field* const* B,
+                          const int64_t* offsets, const double* xv0, double* out_u,
+                          double* out_t, int32_t* out_len, int32_t* retcode,
+                          tpamd_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -18,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 using namespace tpamd;
 
@@ -27,6 +29,7 @@ namespace {
 // Precision policies
 // ---------------------------------------------------------------------------
 struct S64 {
+    static constexpr int kNS = -1;   // generic: any algorithm, adaptive or not
     using R = double;
     struct Pos { double x[3]; };
     Pos pos;
@@ -77,6 +80,7 @@ struct S64 {
 };
 
 struct S32C {
+    static constexpr int kNS = -1;
     using R = float;
     using Pos = Pos32;
     Pos pos;
@@ -398,8 +402,9 @@ __global__ void __launch_bounds__(256) k_adaptive(const KParams p)
 // Specialised gridded-B policies (see tpamd_grid.h).  NS = 0 is the Boris
 // update, NS >= 1 the rational multistep update with n = NS subcycles.
 // ---------------------------------------------------------------------------
-template <int EMODE, int NS, bool PER, typename T>
+template <int EMODE, int NS, bool PER, typename T, bool CELLS>
 struct S64G : S64 {
+    static constexpr int kNS = NS;
     __device__ void bfield(const KParams& p, double B[3]) const
     {
         int i0[3], i1[3];
@@ -412,9 +417,13 @@ struct S64G : S64 {
                 return;
             }
         }
-        unsigned ox[2], oy[2], oz[2];
-        offsets(p.B, i0, i1, sizeof(T), ox, oy, oz);
-        trilinear<T, double>(p.B.data, ox, oy, oz, w, B);
+        if constexpr (CELLS) {
+            trilinear_cell<T, double>(p.B.cells, cell_offset(p.B, i0), w, B);
+        } else {
+            unsigned ox[2], oy[2], oz[2];
+            offsets(p.B, i0, i1, sizeof(T), ox, oy, oz);
+            trilinear<T, double>(p.B.data, ox, oy, oz, w, B);
+        }
     }
     template <int ALG>
     __device__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
@@ -434,8 +443,9 @@ struct S64G : S64 {
     }
 };
 
-template <int EMODE, int NS, bool PER>
+template <int EMODE, int NS, bool PER, bool CELLS>
 struct S32G : S32C {
+    static constexpr int kNS = NS;
     __device__ void bfield(const KParams& p, float B[3]) const
     {
         int i0[3], i1[3];
@@ -448,9 +458,13 @@ struct S32G : S32C {
                 return;
             }
         }
-        unsigned ox[2], oy[2], oz[2];
-        offsets(p.B, i0, i1, 4u, ox, oy, oz);
-        trilinear<float, float>(p.B.data, ox, oy, oz, w, B);
+        if constexpr (CELLS) {
+            trilinear_cell<float, float>(p.B.cells, cell_offset(p.B, i0), w, B);
+        } else {
+            unsigned ox[2], oy[2], oz[2];
+            offsets(p.B, i0, i1, 4u, ox, oy, oz);
+            trilinear<float, float>(p.B.data, ox, oy, oz, w, B);
+        }
     }
     template <int ALG>
     __device__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
@@ -469,6 +483,53 @@ struct S32G : S32C {
         return (double)sqrtf(dot3f(B, B));
     }
 };
+
+// ---------------------------------------------------------------------------
+// Batched ensembles: a persistent grid of waves pulls chunks of particles from
+// an atomic queue.  A chunk never straddles a field instance, so the
+// instance's parameter block is wave-uniform; within a chunk the 64 lanes
+// stride over its particles (particles per thread = chunk size / 64), which
+// evens out uneven lifetimes.  Every wave leaves once the queue is drained.
+// ---------------------------------------------------------------------------
+struct Chunk {
+    int inst;
+    int pad;
+    long long begin, end;
+};
+
+__global__ void k_batch_reset(unsigned* counter, unsigned long long* steps)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *counter = 0u;
+        *steps = 0ull;
+    }
+}
+
+template <class S, int ALG, bool ADAPT>
+__global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table,
+                                               const Chunk* __restrict__ chunks, int nchunks,
+                                               unsigned* __restrict__ counter,
+                                               unsigned long long* __restrict__ steps_out)
+{
+    const int lane = threadIdx.x & 63;
+    long long steps = 0;
+    for (;;) {
+        unsigned c = 0;
+        if (lane == 0) c = atomicAdd(counter, 1u);
+        c = __shfl(c, 0, 64);
+        if (c >= (unsigned)nchunks) break;
+        const Chunk ch = chunks[c];
+        const KParams& p = table[ch.inst];
+        for (long long i = ch.begin + lane; i < ch.end; i += 64) {
+            if constexpr (ADAPT) steps += run_adaptive<S, ALG>(p, i);
+            else steps += run_fixed<S, ALG>(p, i);
+        }
+    }
+    unsigned long long v = (unsigned long long)steps;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) atomicAdd(steps_out, v);
+}
 
 __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const double* __restrict__ x,
                                                     long long n, double t, int which, int prec,
@@ -503,6 +564,7 @@ __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const doubl
 struct tpamd_field {
     DevField d;
     void* dev_data;          // owned device buffer (grids)
+    void* cells_data;        // owned cell-replicated copy (grids, optional)
     const tpamd_field* src;  // UxB: the B field it was built from
     size_t bytes;
 };
@@ -661,101 +723,163 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
     return TPAMD_OK;
 }
 
-template <class S>
-void launch_solve(const KParams& p, int adaptive, int alg, dim3 grid, dim3 block, hipStream_t st)
+// ---------------------------------------------------------------------------
+// Policy selection.  select_policy calls l.go<S, ALG>() with the specialised
+// gridded-B policy that matches the solve, or with the generic policy.  The
+// specialisations cover: B a grid whose bytes fit 32-bit offsets, E zero,
+// uniform or UxB, the Boris update (fixed or adaptive) or the rational
+// multistep update with n <= 2 (fixed step), FP32C on float storage.
+// ---------------------------------------------------------------------------
+template <class L, int EMODE, int NS, bool CELLS>
+void sel_g32(const L& l, bool per)
 {
-    if (adaptive) {
-        if (alg == 0) hipLaunchKernelGGL((k_adaptive<S, 0>), grid, block, 0, st, p);
-        else hipLaunchKernelGGL((k_adaptive<S, 1>), grid, block, 0, st, p);
-    } else {
-        if (alg == 0) hipLaunchKernelGGL((k_fixed<S, 0>), grid, block, 0, st, p);
-        else hipLaunchKernelGGL((k_fixed<S, 1>), grid, block, 0, st, p);
-    }
+    if (per) l.template go<S32G<EMODE, NS, true, CELLS>, NS == 0 ? 0 : 1>();
+    else l.template go<S32G<EMODE, NS, false, CELLS>, NS == 0 ? 0 : 1>();
 }
 
-// one specialised policy: fixed step for every NS, adaptive for Boris (NS = 0)
-template <class S, int NS>
-void launch_spec(const KParams& p, int adaptive, dim3 g, dim3 b, hipStream_t st)
+template <class L, int EMODE, int NS, typename T, bool CELLS>
+void sel_g64(const L& l, bool per)
 {
-    if constexpr (NS == 0) {
-        if (adaptive) {
-            hipLaunchKernelGGL((k_adaptive<S, 0>), g, b, 0, st, p);
-            return;
-        }
-    }
-    hipLaunchKernelGGL((k_fixed<S, NS == 0 ? 0 : 1>), g, b, 0, st, p);
+    if (per) l.template go<S64G<EMODE, NS, true, T, CELLS>, NS == 0 ? 0 : 1>();
+    else l.template go<S64G<EMODE, NS, false, T, CELLS>, NS == 0 ? 0 : 1>();
 }
 
-template <int EMODE, int NS>
-void launch_g32(const KParams& p, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
+template <class L, int EMODE, bool CELLS>
+void sel_g32_ns(const L& l, int ns, bool per)
 {
-    if (per) launch_spec<S32G<EMODE, NS, true>, NS>(p, adaptive, g, b, st);
-    else launch_spec<S32G<EMODE, NS, false>, NS>(p, adaptive, g, b, st);
+    if (ns == 0) sel_g32<L, EMODE, 0, CELLS>(l, per);
+    else if (ns == 1) sel_g32<L, EMODE, 1, CELLS>(l, per);
+    else sel_g32<L, EMODE, 2, CELLS>(l, per);
 }
 
-template <int EMODE, int NS, typename T>
-void launch_g64(const KParams& p, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
+template <class L, int EMODE, typename T, bool CELLS>
+void sel_g64_ns(const L& l, int ns, bool per)
 {
-    if (per) launch_spec<S64G<EMODE, NS, true, T>, NS>(p, adaptive, g, b, st);
-    else launch_spec<S64G<EMODE, NS, false, T>, NS>(p, adaptive, g, b, st);
+    if (ns == 0) sel_g64<L, EMODE, 0, T, CELLS>(l, per);
+    else if (ns == 1) sel_g64<L, EMODE, 1, T, CELLS>(l, per);
+    else sel_g64<L, EMODE, 2, T, CELLS>(l, per);
 }
 
-template <int EMODE, typename T>
-void launch_g64_ns(const KParams& p, int ns, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
+// -1 when no specialisation applies, else the subcycle class NS (0 = Boris)
+int spec_class(const tpamd_solve_desc* d, const KParams& p, int* emode)
 {
-    if (ns == 0) launch_g64<EMODE, 0, T>(p, per, adaptive, g, b, st);
-    else if (ns == 1) launch_g64<EMODE, 1, T>(p, per, adaptive, g, b, st);
-    else launch_g64<EMODE, 2, T>(p, per, adaptive, g, b, st);
-}
-
-template <int EMODE>
-void launch_g32_ns(const KParams& p, int ns, bool per, int adaptive, dim3 g, dim3 b, hipStream_t st)
-{
-    if (ns == 0) launch_g32<EMODE, 0>(p, per, adaptive, g, b, st);
-    else if (ns == 1) launch_g32<EMODE, 1>(p, per, adaptive, g, b, st);
-    else launch_g32<EMODE, 2>(p, per, adaptive, g, b, st);
-}
-
-// Launches a specialised kernel when the solve is a gridded-B one it covers;
-// returns false (nothing launched) otherwise.
-bool launch_specialised(const KParams& p, const tpamd_solve_desc* d, const tpamd_field* E,
-                        const tpamd_field* B, dim3 g, dim3 b, hipStream_t st)
-{
-    if (d->flags & TPAMD_FLAG_GENERIC_KERNELS) return false;
-    if (B->d.kind != kFieldGrid) return false;
-    int emode;
-    if (E->d.kind == kFieldUxB) emode = kEUxB;        // setup_fields checked E->src == B
-    else if (E->d.kind == kFieldZero || E->d.kind == kFieldUniform) emode = kEUniform;
-    else return false;
+    if (d->flags & TPAMD_FLAG_GENERIC_KERNELS) return -1;
+    if (p.B.kind != kFieldGrid) return -1;
+    if (p.e_is_uxb) *emode = kEUxB;
+    else if (p.E.kind == kFieldZero || p.E.kind == kFieldUniform) *emode = kEUniform;
+    else return -1;
     int ns;
     if (d->alg == TPAMD_ALG_BORIS) ns = 0;
     else if (d->form == TPAMD_FORM_RATIONAL && (d->nsub == 1 || d->nsub == 2)) ns = d->nsub;
-    else return false;
-    if (d->adaptive && ns != 0) return false;
-    const double es = B->d.dtype == 1 ? 4.0 : 8.0;
-    if (3.0 * es * (double)B->d.n[0] * (double)B->d.n[1] * (double)B->d.n[2] >= 4294967296.0)
-        return false;                                  // 32-bit byte offsets
-    const bool per = B->d.extrap == kExtrapPeriodic;
-    if (d->precision == TPAMD_FP32C) {
-        if (B->d.dtype != 1) return false;
-        if (emode == kEUxB) launch_g32_ns<kEUxB>(p, ns, per, d->adaptive, g, b, st);
-        else launch_g32_ns<kEUniform>(p, ns, per, d->adaptive, g, b, st);
-    } else if (B->d.dtype == 1) {
-        if (emode == kEUxB) launch_g64_ns<kEUxB, float>(p, ns, per, d->adaptive, g, b, st);
-        else launch_g64_ns<kEUniform, float>(p, ns, per, d->adaptive, g, b, st);
-    } else {
-        if (emode == kEUxB) launch_g64_ns<kEUxB, double>(p, ns, per, d->adaptive, g, b, st);
-        else launch_g64_ns<kEUniform, double>(p, ns, per, d->adaptive, g, b, st);
-    }
-    return true;
+    else return -1;
+    if (d->adaptive && ns != 0) return -1;
+    const double es = p.B.dtype == 1 ? 4.0 : 8.0;
+    if (3.0 * es * (double)p.B.n[0] * (double)p.B.n[1] * (double)p.B.n[2] >= 4294967296.0)
+        return -1;
+    if (d->precision == TPAMD_FP32C && p.B.dtype != 1) return -1;
+    return ns;
 }
 
-int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* E,
-               const tpamd_field* B, const double* d_xv0, int64_t N, double* d_out_u,
-               double* d_out_t, int32_t* d_len, int32_t* d_ret, hipStream_t st,
-               tpamd_stats* stats)
+template <class L>
+void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec, const L& l)
 {
-    int64_t nt = 0, nout = 0;
+    int emode = 0;
+    const int ns = allow_spec ? spec_class(d, p, &emode) : -1;
+    if (ns >= 0) {
+        const bool per = p.B.extrap == kExtrapPeriodic;
+        const bool cells = p.B.cells != nullptr;
+        if (d->precision == TPAMD_FP32C) {
+            if (cells) {
+                if (emode == kEUxB) sel_g32_ns<L, kEUxB, true>(l, ns, per);
+                else sel_g32_ns<L, kEUniform, true>(l, ns, per);
+            } else {
+                if (emode == kEUxB) sel_g32_ns<L, kEUxB, false>(l, ns, per);
+                else sel_g32_ns<L, kEUniform, false>(l, ns, per);
+            }
+        } else if (p.B.dtype == 1) {
+            if (emode == kEUxB) sel_g64_ns<L, kEUxB, float, false>(l, ns, per);
+            else sel_g64_ns<L, kEUniform, float, false>(l, ns, per);
+        } else if (cells) {
+            if (emode == kEUxB) sel_g64_ns<L, kEUxB, double, true>(l, ns, per);
+            else sel_g64_ns<L, kEUniform, double, true>(l, ns, per);
+        } else {
+            if (emode == kEUxB) sel_g64_ns<L, kEUxB, double, false>(l, ns, per);
+            else sel_g64_ns<L, kEUniform, double, false>(l, ns, per);
+        }
+        return;
+    }
+    if (d->precision == TPAMD_FP64) {
+        if (p.alg == 0) l.template go<S64, 0>();
+        else l.template go<S64, 1>();
+    } else {
+        if (p.alg == 0) l.template go<S32C, 0>();
+        else l.template go<S32C, 1>();
+    }
+}
+
+// one solve: a thread per particle
+struct SoloLaunch {
+    const KParams& p;
+    int adaptive;
+    dim3 g, b;
+    hipStream_t st;
+    template <class S, int ALG>
+    void go() const
+    {
+        if constexpr (S::kNS <= 0) {
+            if (adaptive) {
+                hipLaunchKernelGGL((k_adaptive<S, ALG>), g, b, 0, st, p);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_fixed<S, ALG>), g, b, 0, st, p);
+    }
+};
+
+// batched solve: persistent waves over the chunk queue.  With `occ` set it
+// only reports the kernel's resident blocks per CU for block size b.x.
+struct BatchLaunch {
+    const KParams* table;
+    const Chunk* chunks;
+    int nchunks;
+    unsigned* counter;
+    unsigned long long* steps;
+    int adaptive;
+    dim3 g, b;
+    hipStream_t st;
+    int* occ;
+    template <class K>
+    void run(K kern) const
+    {
+        if (occ) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, kern, (int)b.x, 0) != hipSuccess)
+                *occ = 1;
+            return;
+        }
+        hipLaunchKernelGGL(kern, g, b, 0, st, table, chunks, nchunks, counter, steps);
+    }
+    template <class S, int ALG>
+    void go() const
+    {
+        if constexpr (S::kNS <= 0) {
+            if (adaptive) {
+                run(k_batch<S, ALG, true>);
+                return;
+            }
+        }
+        run(k_batch<S, ALG, false>);
+    }
+};
+
+// Validates the descriptor and fills the kernel parameter block (everything
+// but the launch).  Shared by single and batched solves.
+int fill_params(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* E,
+                const tpamd_field* B, const double* d_xv0, int64_t N, double* d_out_u,
+                double* d_out_t, int32_t* d_len, int32_t* d_ret, KParams& p, int64_t& nt,
+                int64_t& nout)
+{
+    nt = 0;
+    nout = 0;
     int rc = check_desc(ctx, desc, &nt, &nout);
     if (rc) return rc;
     if (N < 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "N must be >= 0");
@@ -763,7 +887,6 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
         return fail(ctx, TPAMD_ERR_ARGUMENT, "nout_cap (%lld) < records per particle (%lld)",
                     (long long)desc->nout_cap, (long long)nout);
     if (desc->nout_cap < 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "nout_cap must be >= 0");
-    KParams p;
     std::memset(&p, 0, sizeof p);
     rc = setup_fields(ctx, E, B, desc->precision, p);
     if (rc) return rc;
@@ -811,6 +934,18 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     p.r_in = desc->r_in;
     p.r_out = desc->r_out;
     for (int d = 0; d < 3; ++d) { p.box_lo[d] = desc->box_lo[d]; p.box_hi[d] = desc->box_hi[d]; }
+    if (desc->bnd_flags & TPAMD_BND_GRID) {
+        // open boundary at the faces of B's grid: the node span becomes the box
+        if (B->d.kind != kFieldGrid)
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "TPAMD_BND_GRID needs a gridded B field");
+        for (int d = 0; d < 3; ++d) {
+            const double lo = B->d.origin[d];
+            const double hi = B->d.origin[d] + (double)(B->d.n[d] - 1) / B->d.inv_h[d];
+            if (!(desc->bnd_flags & TPAMD_BND_BOX)) { p.box_lo[d] = lo; p.box_hi[d] = hi; }
+            else { p.box_lo[d] = std::fmax(p.box_lo[d], lo); p.box_hi[d] = std::fmin(p.box_hi[d], hi); }
+        }
+        p.bnd_flags = (p.bnd_flags & ~TPAMD_BND_GRID) | TPAMD_BND_BOX;
+    }
     p.xv0 = d_xv0;
     p.dt_pp = desc->dt_pp;
     p.out_u = d_out_u;
@@ -818,7 +953,18 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     p.out_len = d_len;
     p.retcode = d_ret;
     p.steps = ctx->d_steps;
+    return TPAMD_OK;
+}
 
+int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* E,
+               const tpamd_field* B, const double* d_xv0, int64_t N, double* d_out_u,
+               double* d_out_t, int32_t* d_len, int32_t* d_ret, hipStream_t st,
+               tpamd_stats* stats)
+{
+    KParams p;
+    int64_t nt = 0, nout = 0;
+    int rc = fill_params(ctx, desc, E, B, d_xv0, N, d_out_u, d_out_t, d_len, d_ret, p, nt, nout);
+    if (rc) return rc;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_steps, 0, sizeof(unsigned long long), st));
     const int bs = desc->block_size > 0 ? desc->block_size : 256;
@@ -828,10 +974,7 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     if (N > 0) {
         const dim3 block(bs), grid((unsigned)((N + bs - 1) / bs));
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-        if (!launch_specialised(p, desc, E, B, grid, block, st)) {
-            if (desc->precision == TPAMD_FP64) launch_solve<S64>(p, desc->adaptive, p.alg, grid, block, st);
-            else launch_solve<S32C>(p, desc->adaptive, p.alg, grid, block, st);
-        }
+        select_policy(desc, p, true, SoloLaunch{p, desc->adaptive, grid, block, st});
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     }
@@ -853,6 +996,120 @@ int64_t max_len_host(const int32_t* len, int64_t N)
     int64_t m = 0;
     for (int64_t i = 0; i < N; ++i) m = len[i] > m ? len[i] : m;
     return m;
+}
+
+}  // namespace
+
+struct tpamd_batch {
+    tpamd_ctx* ctx;
+    tpamd_solve_desc desc;
+    int n_inst;
+    int64_t N;
+    std::vector<KParams> table;
+    std::vector<int64_t> offsets;
+    KParams* d_table;
+    Chunk* d_chunks;
+    int nchunks;
+    size_t chunks_cap;
+    unsigned* d_counter;
+    unsigned long long* d_steps;
+    bool allow_spec;
+    int block_size;
+    int ppt;
+    int n_cu;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    bool have_graph;
+};
+
+namespace {
+
+int batch_chunks(tpamd_batch* b, int ppt, int stride)
+{
+    std::vector<Chunk> ch;
+    const int64_t csz = 64 * (int64_t)ppt;
+    int64_t k = 0;
+    for (int i = 0; i < b->n_inst; ++i)
+        for (int64_t s = b->offsets[i]; s < b->offsets[i + 1]; s += csz, ++k)
+            if (k % stride == 0) {
+                Chunk c;
+                c.inst = i;
+                c.pad = 0;
+                c.begin = s;
+                c.end = std::min(s + csz, b->offsets[i + 1]);
+                ch.push_back(c);
+            }
+    tpamd_ctx* ctx = b->ctx;
+    if (ch.size() > b->chunks_cap) {
+        if (b->d_chunks) HIP_TRY(ctx, hipFree(b->d_chunks));
+        b->d_chunks = nullptr;
+        b->chunks_cap = 0;
+        HIP_TRY(ctx, hipMalloc((void**)&b->d_chunks, sizeof(Chunk) * std::max<size_t>(ch.size(), 1)));
+        b->chunks_cap = ch.size();
+    }
+    if (!ch.empty())
+        HIP_TRY(ctx, hipMemcpy(b->d_chunks, ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice));
+    b->nchunks = (int)ch.size();
+    return TPAMD_OK;
+}
+
+// enqueue reset + persistent launch on `st`
+int batch_enqueue(tpamd_batch* b, hipStream_t st)
+{
+    tpamd_ctx* ctx = b->ctx;
+    const dim3 block(b->block_size);
+    int occ = 0;
+    BatchLaunch q{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
+                  dim3(1), block, st, &occ};
+    select_policy(&b->desc, b->table[0], b->allow_spec, q);
+    occ = std::max(occ, 1);
+    const long long waves_needed = b->nchunks;
+    long long blocks = (long long)b->n_cu * occ;
+    const long long waves_per_block = b->block_size / 64;
+    blocks = std::min(blocks, (waves_needed + waves_per_block - 1) / waves_per_block);
+    blocks = std::max(blocks, 1LL);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counter, b->d_steps);
+    BatchLaunch l{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
+                  dim3((unsigned)blocks), block, st, nullptr};
+    select_policy(&b->desc, b->table[0], b->allow_spec, l);
+    HIP_TRY(ctx, hipGetLastError());
+    return TPAMD_OK;
+}
+
+int batch_capture(tpamd_batch* b)
+{
+    tpamd_ctx* ctx = b->ctx;
+    if (b->have_graph) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->have_graph = false;
+    }
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue(b, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, TPAMD_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->have_graph = true;
+    return TPAMD_OK;
+}
+
+int batch_time_once(tpamd_batch* b, hipStream_t st, float* ms)
+{
+    tpamd_ctx* ctx = b->ctx;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    int rc = batch_enqueue(b, st);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    HIP_TRY(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return TPAMD_OK;
 }
 
 }  // namespace
@@ -911,6 +1168,7 @@ static int32_t new_field(tpamd_ctx* ctx, tpamd_field** out, int kind)
     std::memset(&f->d, 0, sizeof f->d);
     f->d.kind = kind;
     f->dev_data = nullptr;
+    f->cells_data = nullptr;
     f->src = nullptr;
     f->bytes = 0;
     *out = f;
@@ -1031,11 +1289,55 @@ int32_t tpamd_field_uxb(tpamd_ctx* ctx, const double u[3], const tpamd_field* B,
 int32_t tpamd_field_free(tpamd_ctx* ctx, tpamd_field* f)
 {
     if (!f) return TPAMD_OK;
-    if (f->dev_data) {
-        if (ctx) (void)hipSetDevice(ctx->device);
-        (void)hipFree(f->dev_data);
-    }
+    if (ctx) (void)hipSetDevice(ctx->device);
+    if (f->dev_data) (void)hipFree(f->dev_data);
+    if (f->cells_data) (void)hipFree(f->cells_data);
     delete f;
+    return TPAMD_OK;
+}
+
+int32_t tpamd_field_grid_cells(tpamd_ctx* ctx, tpamd_field* f, int32_t enable)
+{
+    if (!ctx || !f) return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
+    if (f->d.kind != kFieldGrid) return fail(ctx, TPAMD_ERR_ARGUMENT, "not a gridded field");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (!enable) {
+        if (f->cells_data) HIP_TRY(ctx, hipFree(f->cells_data));
+        f->cells_data = nullptr;
+        f->d.cells = nullptr;
+        return TPAMD_OK;
+    }
+    if (f->d.cells) return TPAMD_OK;
+    const bool per = f->d.extrap == kExtrapPeriodic;
+    int nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = per ? f->d.n[d] : f->d.n[d] - 1;
+    const bool f32 = f->d.dtype == 1;
+    const unsigned stride = f32 ? 128u : 192u;
+    const double total = (double)nc[0] * nc[1] * nc[2];
+    if (total * stride >= 4294967296.0)
+        return fail(ctx, TPAMD_ERR_UNSUPPORTED, "cell layout needs fewer than 4 GiB of cells");
+    void* cells = nullptr;
+    HIP_TRY(ctx, hipMalloc(&cells, (size_t)(total * stride)));
+    const long long ncell = (long long)total;
+    const unsigned blocks = (unsigned)((ncell + 255) / 256);
+    if (f32)
+        hipLaunchKernelGGL(k_build_cells<float>, dim3(blocks), dim3(256), 0, ctx->stream,
+                           (const float*)f->d.data, f->d.n[0], f->d.n[1], f->d.n[2], (int)per, nc[0],
+                           nc[1], nc[2], stride / 4u, (float*)cells);
+    else
+        hipLaunchKernelGGL(k_build_cells<double>, dim3(blocks), dim3(256), 0, ctx->stream,
+                           (const double*)f->d.data, f->d.n[0], f->d.n[1], f->d.n[2], (int)per, nc[0],
+                           nc[1], nc[2], stride / 8u, (double*)cells);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(cells);
+        return fail(ctx, TPAMD_ERR_DEVICE, "cell layout build failed: %s", hipGetErrorString(e));
+    }
+    f->cells_data = cells;
+    f->d.cells = cells;
+    for (int d = 0; d < 3; ++d) f->d.nc[d] = nc[d];
+    f->d.cell_stride = stride;
     return TPAMD_OK;
 }
 
@@ -1141,6 +1443,221 @@ int32_t tpamd_solve(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_fi
         dd.dt_pp = d_dt;
     }
     int rc = solve_impl(ctx, &dd, E, B, d_xv, N, d_u, d_t, d_len, d_ret, st, stats);
+    if (rc) return rc;
+    if (N > 0) {
+        if (b_u) HIP_TRY(ctx, hipMemcpyAsync(out_u, d_u, b_u, hipMemcpyDeviceToHost, st));
+        if (b_t) HIP_TRY(ctx, hipMemcpyAsync(out_t, d_t, b_t, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(out_len, d_len, b_i, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(retcode, d_ret, b_i, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+    }
+    const int64_t ml = N > 0 ? max_len_host(out_len, N) : 0;
+    if (stats) {
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->max_len = ml;
+    }
+    if (desc->adaptive && ml > cap)
+        return fail(ctx, TPAMD_ERR_CAPACITY, "adaptive solve produced %lld records for a capacity of %lld",
+                    (long long)ml, (long long)cap);
+    return TPAMD_OK;
+}
+
+int32_t tpamd_batch_free(tpamd_batch* b)
+{
+    if (!b) return TPAMD_OK;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->have_graph) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+    }
+    if (b->d_table) (void)hipFree(b->d_table);
+    if (b->d_chunks) (void)hipFree(b->d_chunks);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    delete b;
+    return TPAMD_OK;
+}
+
+int32_t tpamd_batch_create(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t n_inst,
+                           const tpamd_field* const* E, const tpamd_field* const* B,
+                           const int64_t* offsets, const double* d_xv0, double* d_out_u,
+                           double* d_out_t, int32_t* d_out_len, int32_t* d_retcode,
+                           tpamd_batch** out)
+{
+    if (!ctx) return fail(nullptr, TPAMD_ERR_ARGUMENT, "null context");
+    if (!desc || !E || !B || !offsets || !out || n_inst < 1)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "batch needs a descriptor, n_inst >= 1 fields and offsets");
+    if (desc->dt_pp) return fail(ctx, TPAMD_ERR_UNSUPPORTED, "per-particle dt is not supported in batches");
+    *out = nullptr;
+    if (offsets[0] != 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "offsets[0] must be 0");
+    for (int i = 0; i < n_inst; ++i)
+        if (offsets[i + 1] < offsets[i]) return fail(ctx, TPAMD_ERR_ARGUMENT, "offsets must be non-decreasing");
+    const int64_t N = offsets[n_inst];
+    if (N > 0 && (!d_xv0 || !d_out_len || !d_retcode || (desc->nout_cap > 0 && (!d_out_u || !d_out_t))))
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "null buffer");
+    tpamd_batch* b = new tpamd_batch();
+    b->ctx = ctx;
+    b->desc = *desc;
+    b->n_inst = n_inst;
+    b->N = N;
+    b->offsets.assign(offsets, offsets + n_inst + 1);
+    b->table.resize(n_inst);
+    b->d_table = nullptr;
+    b->d_chunks = nullptr;
+    b->nchunks = 0;
+    b->chunks_cap = 0;
+    b->d_counter = nullptr;
+    b->d_steps = nullptr;
+    b->block_size = desc->block_size > 0 ? desc->block_size : 256;
+    b->ppt = 4;
+    b->have_graph = false;
+    if (b->block_size % 64 != 0 || b->block_size > 256) {
+        tpamd_batch_free(b);
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "block_size must be a multiple of 64 up to 256");
+    }
+    int64_t nt, nout;
+    for (int i = 0; i < n_inst; ++i) {
+        const int rc = fill_params(ctx, desc, E[i], B[i], d_xv0, N, d_out_u, d_out_t, d_out_len,
+                                   d_retcode, b->table[i], nt, nout);
+        if (rc) {
+            tpamd_batch_free(b);
+            return rc;
+        }
+    }
+    // a specialised policy only if every instance maps to the same one
+    int em0 = 0;
+    const int ns0 = spec_class(desc, b->table[0], &em0);
+    b->allow_spec = ns0 >= 0;
+    for (int i = 1; i < n_inst && b->allow_spec; ++i) {
+        int em = 0;
+        const int ns = spec_class(desc, b->table[i], &em);
+        b->allow_spec = ns == ns0 && em == em0 && b->table[i].B.dtype == b->table[0].B.dtype &&
+                        (b->table[i].B.extrap == kExtrapPeriodic) == (b->table[0].B.extrap == kExtrapPeriodic) &&
+                        (b->table[i].B.cells != nullptr) == (b->table[0].B.cells != nullptr);
+    }
+    hipDeviceProp_t prop;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_table, sizeof(KParams) * n_inst);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_counter, 64);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_table, b->table.data(), sizeof(KParams) * n_inst, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        tpamd_batch_free(b);
+        return fail(ctx, TPAMD_ERR_DEVICE, "batch set-up failed: %s", hipGetErrorString(e));
+    }
+    b->d_steps = (unsigned long long*)((char*)b->d_counter + 8);
+    b->n_cu = prop.multiProcessorCount;
+    int rc = batch_chunks(b, b->ppt, 1);
+    if (!rc) rc = batch_capture(b);
+    if (rc) {
+        tpamd_batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return TPAMD_OK;
+}
+
+int32_t tpamd_batch_run(tpamd_batch* b, void* stream, tpamd_stats* stats)
+{
+    if (!b) return fail(nullptr, TPAMD_ERR_ARGUMENT, "null batch");
+    tpamd_ctx* ctx = b->ctx;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    HIP_TRY(ctx, hipGraphLaunch(b->exec, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    unsigned long long steps = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&steps, b->d_steps, sizeof steps, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->particle_steps = (int64_t)steps;
+        stats->nt = b->table[0].nt;
+        stats->nout = b->table[0].nout;
+        stats->max_len = -1;
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return TPAMD_OK;
+}
+
+int32_t tpamd_batch_autotune(tpamd_batch* b, void* stream, int32_t* block_size, int32_t* ppt)
+{
+    if (!b) return fail(nullptr, TPAMD_ERR_ARGUMENT, "null batch");
+    tpamd_ctx* ctx = b->ctx;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // time each candidate on every 8th chunk of the full chunk list
+    static const int kBlocks[] = {64, 128, 256};
+    static const int kPpt[] = {1, 2, 4, 8, 16, 32};
+    float best = 3.0e38f;
+    int best_bs = b->block_size, best_ppt = b->ppt;
+    for (int bs : kBlocks) {
+        for (int pp : kPpt) {
+            b->block_size = bs;
+            int rc = batch_chunks(b, pp, 8);
+            float ms = 0.f;
+            if (!rc) rc = batch_time_once(b, st, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                best_bs = bs;
+                best_ppt = pp;
+            }
+        }
+    }
+    b->block_size = best_bs;
+    b->ppt = best_ppt;
+    int rc = batch_chunks(b, best_ppt, 1);
+    if (!rc) rc = batch_capture(b);
+    if (rc) return rc;
+    if (block_size) *block_size = best_bs;
+    if (ppt) *ppt = best_ppt;
+    return TPAMD_OK;
+}
+
+int32_t tpamd_solve_batch(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t n_inst,
+                          const tpamd_field* const* E, const tpamd_field* const* B,
+                          const int64_t* offsets, const double* xv0, double* out_u,
+                          double* out_t, int32_t* out_len, int32_t* retcode, tpamd_stats* stats)
+{
+    if (!ctx) return fail(nullptr, TPAMD_ERR_ARGUMENT, "null context");
+    if (!desc || !offsets || n_inst < 1) return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
+    const int64_t N = offsets[n_inst];
+    if (N < 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "bad offsets");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t cap = desc->nout_cap;
+    const size_t b_xv = sizeof(double) * 6 * (size_t)N;
+    const size_t b_u = sizeof(double) * 6 * (size_t)cap * (size_t)N;
+    const size_t b_t = sizeof(double) * (size_t)cap * (size_t)N;
+    const size_t b_i = sizeof(int32_t) * (size_t)N;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t need = al(b_xv) + al(b_u) + al(b_t) + 2 * al(b_i);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (need > ctx->d_buf_bytes) {
+        if (ctx->d_buf) HIP_TRY(ctx, hipFree(ctx->d_buf));
+        ctx->d_buf = nullptr;
+        ctx->d_buf_bytes = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_buf, need));
+        ctx->d_buf_bytes = need;
+    }
+    char* p = (char*)ctx->d_buf;
+    double* d_xv = (double*)p; p += al(b_xv);
+    double* d_u = (double*)p; p += al(b_u);
+    double* d_t = (double*)p; p += al(b_t);
+    int32_t* d_len = (int32_t*)p; p += al(b_i);
+    int32_t* d_ret = (int32_t*)p;
+    hipStream_t st = ctx->stream;
+    if (N > 0) HIP_TRY(ctx, hipMemcpyAsync(d_xv, xv0, b_xv, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    tpamd_batch* b = nullptr;
+    int rc = tpamd_batch_create(ctx, desc, n_inst, E, B, offsets, d_xv, d_u, d_t, d_len, d_ret, &b);
+    if (rc) return rc;
+    rc = tpamd_batch_run(b, st, stats);
+    tpamd_batch_free(b);
     if (rc) return rc;
     if (N > 0) {
         if (b_u) HIP_TRY(ctx, hipMemcpyAsync(out_u, d_u, b_u, hipMemcpyDeviceToHost, st));

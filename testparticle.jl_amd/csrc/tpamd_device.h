@@ -43,6 +43,13 @@ struct DevField {
     float inv_hf[3];
     float pad2;
     const void* data;
+    // optional cell-replicated copy: per cell the 8 corner nodes x 3
+    // components, corner (dx, dy, dz) at 3 * (dx + 2 dy + 4 dz), one cell per
+    // `cell_stride` bytes (float: 128 B, one cache line; double: 192 B);
+    // cells per axis nc[d] = n[d] (periodic) or n[d] - 1
+    const void* cells;
+    int nc[3];
+    unsigned cell_stride;
 };
 
 struct Poly {

@@ -124,6 +124,80 @@ __device__ __forceinline__ void trilinear(const void* base, const unsigned ox[2]
     }
 }
 
+// Trilinear blend from one cell-replicated block (24 values, corner
+// (dx, dy, dz) at 3 * (dx + 2 dy + 4 dz)); same lerp order as trilinear().
+template <typename T, typename OUT>
+__device__ __forceinline__ void trilinear_cell(const void* base, unsigned off, const OUT w[3],
+                                               OUT out[3])
+{
+    T v[24];
+    if constexpr (sizeof(T) == 4) {
+        const float4* q = (const float4*)((const char*)base + off);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const float4 a = q[j];
+            v[4 * j] = a.x; v[4 * j + 1] = a.y; v[4 * j + 2] = a.z; v[4 * j + 3] = a.w;
+        }
+    } else {
+        const double2* q = (const double2*)((const char*)base + off);
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+            const double2 a = q[j];
+            v[2 * j] = a.x; v[2 * j + 1] = a.y;
+        }
+    }
+    OUT c[4][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const OUT a = (OUT)v[6 * q + k], b = (OUT)v[6 * q + 3 + k];
+            if constexpr (sizeof(OUT) == 8) c[q][k] = lerp_f64(a, b, w[0]);
+            else c[q][k] = lerp_f32(a, b, w[0]);
+        }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if constexpr (sizeof(OUT) == 8)
+            out[k] = lerp_f64(lerp_f64(c[0][k], c[1][k], w[1]), lerp_f64(c[2][k], c[3][k], w[1]), w[2]);
+        else
+            out[k] = lerp_f32(lerp_f32(c[0][k], c[1][k], w[1]), lerp_f32(c[2][k], c[3][k], w[1]), w[2]);
+    }
+}
+
+__device__ __forceinline__ unsigned cell_offset(const DevField& f, const int i0[3])
+{
+    return f.cell_stride *
+           ((unsigned)i0[0] + (unsigned)f.nc[0] * ((unsigned)i0[1] + (unsigned)f.nc[1] * (unsigned)i0[2]));
+}
+
+// builds the cell-replicated copy from the node array (one thread per cell)
+template <typename T>
+__global__ void k_build_cells(const T* __restrict__ nodes, int nx, int ny, int nz, int periodic,
+                              int ncx, int ncy, int ncz, unsigned stride_elems, T* __restrict__ cells)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)ncx * ncy * ncz;
+    if (c >= total) return;
+    const int i = (int)(c % ncx);
+    const int j = (int)((c / ncx) % ncy);
+    const int k = (int)(c / ((long long)ncx * ncy));
+    T* out = cells + c * stride_elems;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        int a = i + (q & 1), b = j + ((q >> 1) & 1), d = k + (q >> 2);
+        if (periodic) {
+            a = a == nx ? 0 : a;
+            b = b == ny ? 0 : b;
+            d = d == nz ? 0 : d;
+        }
+        const T* src = nodes + 3 * ((long long)a + (long long)nx * ((long long)b + (long long)ny * d));
+        out[3 * q] = src[0];
+        out[3 * q + 1] = src[1];
+        out[3 * q + 2] = src[2];
+    }
+    for (unsigned e = 24; e < stride_elems; ++e) out[e] = (T)0;
+}
+
 __device__ __forceinline__ void offsets(const DevField& f, const int i0[3], const int i1[3],
                                         unsigned es, unsigned ox[2], unsigned oy[2], unsigned oz[2])
 {

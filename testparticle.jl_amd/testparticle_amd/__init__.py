@@ -19,9 +19,9 @@ from .constants import (BMoment_Earth, Electron, Ion, Proton, Re, Species, Speci
 from .fields import (AffineField, ClampExtrap, DipoleField, FieldInterpolator, FillExtrap,
                      GridField, PeriodicBC, UniformField, UxBField, WrapExtrap, ZeroField,
                      build_interpolator, prepare)
-from .problem import (Boundary, EnsembleContext, InsideSphere, OutsideBox, OutsideNaN,
-                      OutsideSphere, TerminateOutside, TraceProblem, remake)
+from .problem import (Boundary, EnsembleContext, InsideSphere, OutsideBox, OutsideGrid,
+                      OutsideNaN, OutsideSphere, TerminateOutside, TraceProblem, remake)
 from .solution import EnsembleSolution, TraceSolution
-from .solve import initial_states, make_desc, plan, solve
+from .solve import initial_states, make_desc, plan, solve, solve_batch
 
 __all__ = [n for n in dir() if not n.startswith("_")]

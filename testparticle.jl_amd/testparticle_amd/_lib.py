@@ -24,7 +24,7 @@ F64, F32 = 0, 1
 ALG_BORIS, ALG_MULTISTEP = 0, 1
 FORM_REFERENCE, FORM_RATIONAL = 0, 1
 FP64, FP32C = 0, 1
-BND_NAN, BND_SPHERE_IN, BND_SPHERE_OUT, BND_BOX = 1, 2, 4, 8
+BND_NAN, BND_SPHERE_IN, BND_SPHERE_OUT, BND_BOX, BND_GRID = 1, 2, 4, 8, 16
 RET_SUCCESS, RET_TERMINATED, RET_MAXITERS = 0, 1, 2
 FLAG_GENERIC_KERNELS = 1
 RETCODE_NAMES = {RET_SUCCESS: "Success", RET_TERMINATED: "Terminated", RET_MAXITERS: "MaxIters"}
@@ -69,8 +69,11 @@ class Stats(C.Structure):
 EXPORTS = [
     "tpamd_abi_version", "tpamd_init", "tpamd_finalize", "tpamd_last_error",
     "tpamd_field_zero", "tpamd_field_uniform", "tpamd_field_affine", "tpamd_field_dipole",
-    "tpamd_field_grid", "tpamd_field_uxb", "tpamd_field_eval", "tpamd_field_free",
+    "tpamd_field_grid", "tpamd_field_grid_cells", "tpamd_field_uxb", "tpamd_field_eval",
+    "tpamd_field_free",
     "tpamd_plan", "tpamd_solve", "tpamd_solve_device",
+    "tpamd_batch_create", "tpamd_batch_autotune", "tpamd_batch_run", "tpamd_batch_free",
+    "tpamd_solve_batch",
 ]
 
 _lib = None
@@ -102,6 +105,7 @@ def load_library(path: str = LIB_PATH):
         L.tpamd_field_grid.argtypes = [vp, vp, C.c_int32, i64, i64, i64, d3, d3, C.c_int32,
                                        C.c_int32, pp]
         L.tpamd_field_uxb.argtypes = [vp, d3, vp, pp]
+        L.tpamd_field_grid_cells.argtypes = [vp, vp, C.c_int32]
         L.tpamd_field_eval.argtypes = [vp, vp, vp, d3, i64, C.c_double, C.c_int32, d3]
         L.tpamd_field_free.argtypes = [vp, vp]
         L.tpamd_plan.argtypes = [vp, C.POINTER(SolveDesc), C.POINTER(i64), C.POINTER(i64)]
@@ -109,6 +113,14 @@ def load_library(path: str = LIB_PATH):
                                   C.POINTER(Stats)]
         L.tpamd_solve_device.argtypes = [vp, C.POINTER(SolveDesc), vp, vp, vp, i64, vp, vp, vp,
                                          vp, vp, C.POINTER(Stats)]
+        vpa = C.POINTER(C.c_void_p)
+        L.tpamd_batch_create.argtypes = [vp, C.POINTER(SolveDesc), C.c_int32, vpa, vpa,
+                                         C.POINTER(i64), vp, vp, vp, vp, vp, pp]
+        L.tpamd_batch_autotune.argtypes = [vp, vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.tpamd_batch_run.argtypes = [vp, vp, C.POINTER(Stats)]
+        L.tpamd_batch_free.argtypes = [vp]
+        L.tpamd_solve_batch.argtypes = [vp, C.POINTER(SolveDesc), C.c_int32, vpa, vpa,
+                                        C.POINTER(i64), vp, vp, vp, vp, vp, C.POINTER(Stats)]
         for name in EXPORTS:
             if name not in ("tpamd_abi_version", "tpamd_last_error"):
                 getattr(L, name).restype = C.c_int32

@@ -64,3 +64,81 @@ class DeviceEnsemble:
     def host(self):
         return (self.u.cpu().numpy(), self.t.cpu().numpy(), self.len.cpu().numpy(),
                 self.ret.cpu().numpy())
+
+
+class DeviceBatch:
+    """Many field instances in one persistent, graph-captured launch
+    (tpamd_batch_*), with states and records resident in HBM."""
+
+    def __init__(self, problems, alg, *, device: int = 0, nout_cap=None, **desc_kw):
+        import torch
+
+        from .solve import _adaptive_cap, initial_states
+
+        self.torch = torch
+        self.ctx = _lib.context(device)
+        self.dev = torch.device(f"cuda:{device}")
+        self.problems = problems
+        fields = [_fields(pr) for pr in problems]
+        states = [initial_states(pr, 1) for pr in problems]
+        self.offsets = np.zeros(len(problems) + 1, dtype=np.int64)
+        self.offsets[1:] = np.cumsum([s.shape[0] for s in states])
+        self.N = int(self.offsets[-1])
+        self.desc = make_desc(problems[0], alg, **desc_kw)
+        if alg.adaptive:
+            cap = nout_cap if nout_cap is not None else _adaptive_cap(
+                desc_kw.get("save_start", True), desc_kw.get("save_everystep", True),
+                desc_kw.get("max_steps", 0), desc_kw.get("savestepinterval", 1))
+        else:
+            cap = plan(self.desc)[1]
+        self.cap = int(cap)
+        self.desc.nout_cap = self.cap
+        self.xv0 = torch.from_numpy(np.ascontiguousarray(np.concatenate(states))).to(self.dev)
+        self.u = torch.empty((self.N, max(self.cap, 1), 6), dtype=torch.float64, device=self.dev)
+        self.t = torch.empty((self.N, max(self.cap, 1)), dtype=torch.float64, device=self.dev)
+        self.len = torch.empty(self.N, dtype=torch.int32, device=self.dev)
+        self.ret = torch.empty(self.N, dtype=torch.int32, device=self.dev)
+        self._keep = fields
+        hE = (C.c_void_p * len(problems))(*[f[0].handle(self.ctx).value for f in fields])
+        hB = (C.c_void_p * len(problems))(*[f[1].handle(self.ctx).value for f in fields])
+        torch.cuda.synchronize(self.dev)
+        h = C.c_void_p()
+        with self.ctx.lock:
+            rc = self.ctx.L.tpamd_batch_create(
+                self.ctx.h, C.byref(self.desc), len(problems), hE, hB,
+                self.offsets.ctypes.data_as(C.POINTER(C.c_int64)), self.xv0.data_ptr(),
+                self.u.data_ptr(), self.t.data_ptr(), self.len.data_ptr(), self.ret.data_ptr(),
+                C.byref(h))
+        self.ctx.check(rc)
+        self.h = h
+
+    def autotune(self):
+        bs, ppt = C.c_int32(), C.c_int32()
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        with self.ctx.lock:
+            rc = self.ctx.L.tpamd_batch_autotune(self.h, C.c_void_p(stream), C.byref(bs), C.byref(ppt))
+        self.ctx.check(rc)
+        return int(bs.value), int(ppt.value)
+
+    def run(self) -> _lib.Stats:
+        st = _lib.Stats()
+        stream = self.torch.cuda.current_stream(self.dev).cuda_stream
+        with self.ctx.lock:
+            rc = self.ctx.L.tpamd_batch_run(self.h, C.c_void_p(stream), C.byref(st))
+        self.ctx.check(rc, st)
+        return st
+
+    def final_states(self) -> np.ndarray:
+        idx = (self.len.long() - 1).clamp(min=0)
+        ar = self.torch.arange(self.N, device=self.dev)
+        return self.u[ar, idx].cpu().numpy()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.tpamd_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        import sys
+        if not sys.is_finalizing():
+            self.close()

@@ -101,6 +101,12 @@ def OutsideBox(lo, hi) -> Boundary:
                     box_hi=tuple(float(a) for a in hi))
 
 
+def OutsideGrid() -> Boundary:
+    """x outside the node span of the solve's gridded B field (each instance's
+    own grid in a batched solve): particles exit at the grid faces."""
+    return Boundary(flags=_lib.BND_GRID)
+
+
 @dataclass
 class TerminateOutside:
     """TerminateOutside(isoutofdomain) (src/utility/utility.jl:589-591): wraps a

@@ -209,3 +209,74 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
         ctx.check(code, st)
         break
     return EnsembleSolution(t, u, ln, rc, st.kernel_ms * 1e-3, stats=st, prob=prob, alg=alg)
+
+
+def _adaptive_cap(save_start, save_everystep, max_steps, savestepinterval):
+    if not save_everystep:
+        return int(save_start) + 1
+    if max_steps > 0:
+        return int(save_start) + max_steps // savestepinterval + 2
+    return 1024
+
+
+def solve_batch(problems, alg: AbstractBoris, *, dt=None, savestepinterval: int = 1,
+                isoutside=None, save_start: bool = True, save_end: bool = True,
+                save_everystep: bool = True, maxiters: int = 1_000_000,
+                precision: str = "fp64", form: str = "rational", max_steps: int = 0,
+                nout_cap=None, kernels: str = "auto", device: int = 0):
+    """Solve many problems, each with its own fields and (N_k, 6) initial
+    states, in one persistent launch (tpamd_solve_batch).  All problems share
+    q/m, tspan and the solver keywords.  Returns one EnsembleSolution per
+    problem.  The reference has no batched entry point: it would call solve
+    once per field (src/boris/boris.jl:88-114)."""
+    if not problems:
+        return []
+    q2m, tspan = float(problems[0].p[0]), tuple(problems[0].tspan)
+    for pr in problems:
+        if float(pr.p[0]) != q2m or tuple(pr.tspan) != tspan:
+            raise _lib.ArgumentError("batched problems must share q/m and tspan")
+    if alg.adaptive and dt is not None:
+        raise TypeError("the adaptive solver takes no dt (src/boris/adaptive_boris.jl:21-33)")
+    fields = [_fields(pr) for pr in problems]
+    states = [initial_states(pr, 1) for pr in problems]
+    counts = [s.shape[0] for s in states]
+    offsets = np.zeros(len(problems) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(counts)
+    xv0 = np.ascontiguousarray(np.concatenate(states) if offsets[-1] else np.zeros((0, 6)))
+    N = int(offsets[-1])
+    desc = make_desc(problems[0], alg, dt=dt, savestepinterval=savestepinterval,
+                     isoutside=isoutside, save_start=save_start, save_end=save_end,
+                     save_everystep=save_everystep, maxiters=maxiters, precision=precision,
+                     form=form, max_steps=max_steps, kernels=kernels)
+    if not alg.adaptive:
+        cap = plan(desc)[1]
+    elif nout_cap is not None:
+        cap = int(nout_cap)
+    else:
+        cap = _adaptive_cap(save_start, save_everystep, max_steps, savestepinterval)
+    ctx = _lib.context(device)
+    hE = (C.c_void_p * len(problems))(*[f[0].handle(ctx).value for f in fields])
+    hB = (C.c_void_p * len(problems))(*[f[1].handle(ctx).value for f in fields])
+    while True:
+        desc.nout_cap = cap
+        u = np.empty((N, max(cap, 0), 6))
+        t = np.empty((N, max(cap, 0)))
+        ln = np.zeros(N, dtype=np.int32)
+        rc = np.zeros(N, dtype=np.int32)
+        st = _lib.Stats()
+        with ctx.lock:
+            code = ctx.L.tpamd_solve_batch(ctx.h, C.byref(desc), len(problems), hE, hB,
+                                           offsets.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           xv0.ctypes.data, u.ctypes.data, t.ctypes.data,
+                                           ln.ctypes.data, rc.ctypes.data, C.byref(st))
+        if code == _lib.ERR_CAPACITY and nout_cap is None:
+            cap = int(st.max_len)
+            continue
+        ctx.check(code, st)
+        break
+    out = []
+    for k, pr in enumerate(problems):
+        a, b = offsets[k], offsets[k + 1]
+        out.append(EnsembleSolution(t[a:b], u[a:b], ln[a:b], rc[a:b], st.kernel_ms * 1e-3,
+                                    stats=st, prob=pr, alg=alg))
+    return out

@@ -137,6 +137,35 @@ def turbulent_box(N: int = 1 << 22, n: int = 256, seed: int = 3, nsteps: int = 5
     return w, A
 
 
+def batched_turbulence(n_inst: int = 64, seed: int = 4, total: int = 20_000_000,
+                       L: float = 1.0e7, B0: float = 5.0e-9, sizes=(32, 64, 96, 128),
+                       dtype=np.float32, safety: float = 0.1, max_steps: int = 2000):
+    """Uneven batch of independent turbulent boxes with open boundaries.
+
+    Instance k: grid edge drawn from `sizes`, a seeded turbulent field as in
+    turbulent_box on nodes spanning [0, L]^3, and a particle count drawn
+    log-uniform in [1e3, 1e6]; the counts are then scaled by one common factor
+    so that they sum to `total`.  Protons 1-100 keV, isotropic, uniform in the
+    box.  The span covers `max_steps` adaptive steps at |B| = B0.
+    Returns (instances, tspan) with instances = [(A, axes, xv0), ...]."""
+    rng = np.random.default_rng(seed)
+    edges = rng.choice(np.asarray(sizes), n_inst)
+    raw = np.exp(rng.uniform(math.log(1.0e3), math.log(1.0e6), n_inst))
+    counts = np.maximum(1, np.round(raw * (total / raw.sum()))).astype(np.int64)
+    inst = []
+    for k in range(n_inst):
+        n = int(edges[k])
+        A = turbulent_field(n, L, B0, 0.5, seed * 1000 + k, dtype=dtype)
+        axes = grid_axes(n, L, periodic=False)
+        N = int(counts[k])
+        x = rng.uniform(0.0, L, size=(N, 3))
+        Ek = rng.uniform(1.0e3, 1.0e5, N) * EV
+        v = isotropic_directions(rng, N) * np.sqrt(2 * Ek / mi)[:, None]
+        inst.append((A, axes, np.ascontiguousarray(np.hstack([x, v]))))
+    Tg = 2 * math.pi * mi / (qi * B0)
+    return inst, (0.0, max_steps * safety * Tg)
+
+
 def grid_axes(n: int, L: float, periodic: bool = True):
     h = L / n if periodic else L / (n - 1)
     ax = np.arange(n) * h

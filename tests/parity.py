@@ -44,7 +44,19 @@ def run_oracle(prob, alg, *, dt=None, savestepinterval=1, isoutside=None, save_s
                   form=O.FORM_RATIONAL if form == "rational" else O.FORM_REFERENCE)
     b = isoutside.condition if hasattr(isoutside, "condition") else isoutside
     if b is not None:
-        kw.update(bnd_flags=b.flags, r_in=b.r_in, r_out=b.r_out, box_lo=b.box_lo, box_hi=b.box_hi)
+        flags, lo, hi = b.flags, list(b.box_lo), list(b.box_hi)
+        if flags & _lib.BND_GRID:
+            # the grid's node span, computed as the library does: origin + (n-1) / (1/h)
+            g = prob.p[3]
+            for d in range(3):
+                glo = float(g.origin[d])
+                ghi = glo + float(g.shape[d] - 1) / (1.0 / float(g.spacing[d]))
+                if flags & _lib.BND_BOX:
+                    lo[d], hi[d] = max(lo[d], glo), min(hi[d], ghi)
+                else:
+                    lo[d], hi[d] = glo, ghi
+            flags = (flags & ~_lib.BND_GRID) | _lib.BND_BOX
+        kw.update(bnd_flags=flags, r_in=b.r_in, r_out=b.r_out, box_lo=lo, box_hi=hi)
     if alg.adaptive:
         kw.update(adaptive=True, safety=alg.safety, max_steps=max_steps,
                   nout_cap=nout_cap if nout_cap is not None else 4096)

@@ -298,6 +298,30 @@ def test_parity_specialised_f32_storage_and_extrap(turb64, extrap):
         assert_bitwise(g, o)
 
 
+@pytest.mark.parametrize("extrap", ["periodic", "clamp", "fill_nan"])
+def test_cell_layout_is_bit_identical(turb64, extrap):
+    # the cell-replicated copy changes where node values live, not the blend:
+    # FP64 stays bit-identical to the oracle, FP32C to its node-layout run
+    w, A, ax, _, _ = turb64
+    bc = {"periodic": TP.PeriodicBC, "clamp": TP.ClampExtrap, "fill_nan": TP.FillExtrap}[extrap]
+    Bc = TP.GridField(A, *ax, bc=bc(), layout="cells")
+    prob = TP.TraceProblem(w.xv0[:512], (0.0, 300 * w.dt),
+                           (w.q2m, TP.mi, TP.UxBField(w.extra["u"], Bc), Bc, TP.ZeroField()))
+    for alg in (TP.Boris(), TP.MultistepBoris(4, n=2)):
+        g = TP.solve(prob, alg, dt=w.dt, savestepinterval=30)
+        assert_bitwise(g, run_oracle(prob, alg, dt=w.dt, savestepinterval=30))
+    A32 = A.astype(np.float32)
+    Bn = TP.GridField(A32, *ax, bc=bc(), layout="nodes")
+    Bc = TP.GridField(A32, *ax, bc=bc(), layout="cells")
+    res = []
+    for B in (Bn, Bc):
+        pr = TP.TraceProblem(w.xv0[:512], (0.0, 300 * w.dt),
+                             (w.q2m, TP.mi, TP.UxBField(w.extra["u"], B), B, TP.ZeroField()))
+        res.append(TP.solve(pr, TP.MultistepBoris(4, n=2), dt=w.dt, savestepinterval=30,
+                            precision="fp32c"))
+    np.testing.assert_array_equal(res[0].u_all, res[1].u_all)
+
+
 def test_fp32c_specialised_matches_generic(turb64):
     w, A, ax, _, _ = turb64
     B = TP.GridField(A.astype(np.float32), *ax, bc=TP.PeriodicBC())

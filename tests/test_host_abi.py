@@ -170,6 +170,29 @@ def test_solve_rejects_host_callables():
         TP.solve(prob, TP.AdaptiveBoris(), dt=0.1)
 
 
+def test_solve_batch_argument_checks():
+    z = TP.ZeroField()
+    a = TP.TraceProblem(np.zeros((2, 6)), (0.0, 1.0), (1.0, 1.0, z, z, z))
+    b = TP.TraceProblem(np.zeros((3, 6)), (0.0, 2.0), (1.0, 1.0, z, z, z))
+    c = TP.TraceProblem(np.zeros((3, 6)), (0.0, 1.0), (2.0, 1.0, z, z, z))
+    with pytest.raises(TP.ArgumentError, match="share"):
+        TP.solve_batch([a, b], TP.Boris(), dt=0.1)
+    with pytest.raises(TP.ArgumentError, match="share"):
+        TP.solve_batch([a, c], TP.Boris(), dt=0.1)
+    assert TP.solve_batch([], TP.Boris(), dt=0.1) == []
+
+
+def test_batched_workload_generator():
+    from testparticle_amd import workloads as W
+    inst, tspan = W.batched_turbulence(n_inst=5, total=5000, sizes=(8, 12), seed=9)
+    assert sum(x.shape[0] for _, _, x in inst) == pytest.approx(5000, abs=5)
+    for A, ax, xv in inst:
+        n = A.shape[1]
+        assert A.shape == (3, n, n, n) and n in (8, 12)
+        assert ax[0][-1] == pytest.approx(1e7) and np.all((xv[:, :3] >= 0) & (xv[:, :3] < 1e7))
+    assert tspan[1] > 0
+
+
 def test_workload_generators_are_seeded():
     from testparticle_amd import workloads as W
     a = W.uniform_gyration(N=16)
