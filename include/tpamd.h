@@ -138,6 +138,7 @@ typedef struct tpamd_stats {
     int64_t nt;               /* fixed-step count (0 for adaptive) */
     int64_t nout;             /* records per particle the schedule asks for (fixed step) */
     int64_t max_len;          /* largest record count of any particle */
+    double aux_ms;            /* second phase where there is one (adjoint: reverse sweep) */
 } tpamd_stats;
 
 /* ---- library / context -------------------------------------------------- */
@@ -207,6 +208,22 @@ int32_t tpamd_solve_device(tpamd_ctx* ctx, const tpamd_solve_desc* desc,
                            const tpamd_field* E, const tpamd_field* B, const double* d_xv0,
                            int64_t N, double* d_out_u, double* d_out_t, int32_t* d_out_len,
                            int32_t* d_retcode, void* stream, tpamd_stats* stats);
+
+/* ---- adjoint ---------------------------------------------------------------- *
+ * Gradient of the trajectory loss L = (1/N) sum_i |x_i(T) - x*_i|^2 (final
+ * positions of a fixed-step standard Boris solve, E = 0, gridded B) with
+ * respect to every B node value and every initial state.  The reference
+ * differentiates with ForwardDiff through the CPU solve (src/utility/utility.jl,
+ * test/test_derivatives.jl); there is no reverse-mode counterpart.
+ *   desc: alg = BORIS, adaptive = 0, precision FP64 or FP32C, no boundary.
+ *   ckpt_interval K: FP64 state checkpoints every K steps; the reverse sweep
+ *     recomputes each K-step segment.
+ *   targets[N][3]; dL_dB in B's node layout [nz][ny][nx][3] (accumulated in
+ *   float); dL_dxv0[N][6].  stats->kernel_ms = forward pass, aux_ms = reverse. */
+int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* B,
+                      const double* xv0, int64_t N, const double* targets,
+                      int32_t ckpt_interval, double* loss, double* dL_dB, double* dL_dxv0,
+                      tpamd_stats* stats);
 
 /* ---- batched ensembles ---------------------------------------------------- *
  * Many field instances with particle populations of uneven size and lifetime

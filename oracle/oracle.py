@@ -74,6 +74,7 @@ def lib():
                                              C.c_int, C.c_int, d3]
         L.orc_multistep_coeffs.argtypes = [C.c_double, C.c_int, C.c_int, d3]
         L.orc_rational_polys.argtypes = [C.c_int, d3, d3, d3, d3, d3, C.POINTER(C.c_int)]
+        L.orc_boris_tangent.argtypes = [C.POINTER(OrcDesc), C.POINTER(OrcField), d3, d3, d3]
         L.orc_field_eval.argtypes = [C.POINTER(OrcField), d3, C.c_double, d3]
         L.orc_calculate_nout.argtypes = [C.c_long, C.c_long, C.c_int, C.c_int, C.c_int]
         L.orc_calculate_nout.restype = C.c_long
@@ -252,3 +253,17 @@ def solve(xv0, E: Field, B: Field, *, q2m, tspan, dt=0.0, alg=ALG_BORIS, adaptiv
     if rc:
         raise ValueError(ERR_NAMES.get(rc, str(rc)))
     return OracleResult(out_u, out_t, ln, rc_arr, float(el[0]))
+
+
+def boris_tangent(xv0, B: Field, *, q2m, tspan, dt):
+    """Final position and its Jacobian d x(T) / d (x0, v0) (3 x 6) of a
+    fixed-step standard Boris solve with E = 0, by forward-mode duals."""
+    d = OrcDesc(alg=ALG_BORIS, q2m=q2m, t0=tspan[0], t1=tspan[1], dt=dt, maxiters=1 << 40,
+                savestepinterval=1, save_start=0, save_end=1, save_everystep=0)
+    xa = np.ascontiguousarray(xv0, dtype=np.float64)
+    xT = np.zeros(3)
+    J = np.zeros(18)
+    rc = lib().orc_boris_tangent(C.byref(d), C.byref(B.st), _dp(xa), _dp(xT), _dp(J))
+    if rc:
+        raise ValueError(ERR_NAMES.get(rc, str(rc)))
+    return xT, J.reshape(3, 6)

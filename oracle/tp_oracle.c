@@ -571,6 +571,117 @@ static void solve_adaptive_one(const ctx_t* c, long i, const double* xv0, double
     oret[i] = ret;
 }
 
+/* ----------------------------------------------------------------------------
+ * Forward-mode derivative of the final position of a fixed-step standard
+ * Boris solve (E = 0, B gridded) with respect to the initial state (x0, v0):
+ * the same operation sequence as solve_fixed_one, carried on dual numbers
+ * with six tangent directions.  Stands in for ForwardDiff through the
+ * reference (src/boris/boris.jl:19-45, :445-478, :514).
+ * ------------------------------------------------------------------------- */
+typedef struct { double v; double d[6]; } dual6;
+
+static inline dual6 dc(double v) { dual6 r; r.v = v; for (int k = 0; k < 6; ++k) r.d[k] = 0.0; return r; }
+static inline dual6 dadd(dual6 a, dual6 b) { for (int k = 0; k < 6; ++k) a.d[k] += b.d[k]; a.v += b.v; return a; }
+static inline dual6 dsub(dual6 a, dual6 b) { for (int k = 0; k < 6; ++k) a.d[k] -= b.d[k]; a.v -= b.v; return a; }
+static inline dual6 dmul(dual6 a, dual6 b)
+{
+    dual6 r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 6; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+static inline dual6 ddiv(dual6 a, dual6 b)
+{
+    dual6 r;
+    r.v = a.v / b.v;
+    for (int k = 0; k < 6; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    return r;
+}
+static inline dual6 dscale(double s, dual6 a) { for (int k = 0; k < 6; ++k) a.d[k] *= s; a.v *= s; return a; }
+/* value rounded like lerp() above, fma(w, b, (1 - w) a); derivative exact */
+static inline dual6 dlerp(dual6 a, dual6 b, dual6 w)
+{
+    dual6 r = dadd(dmul(dsub(dc(1.0), w), a), dmul(w, b));
+    r.v = fma(w.v, b.v, (1.0 - w.v) * a.v);
+    return r;
+}
+
+static void dcross(const dual6 a[3], const dual6 b[3], dual6 o[3])
+{
+    o[0] = dsub(dmul(a[1], b[2]), dmul(a[2], b[1]));
+    o[1] = dsub(dmul(a[2], b[0]), dmul(a[0], b[2]));
+    o[2] = dsub(dmul(a[0], b[1]), dmul(a[1], b[0]));
+}
+
+static void dgrid(const orc_field* f, const dual6 x[3], dual6 out[3])
+{
+    const long n[3] = {f->nx, f->ny, f->nz};
+    long i0[3], i1[3];
+    dual6 w[3];
+    for (int d = 0; d < 3; ++d) {
+        const dual6 s = dscale(1.0 / f->spacing[d], dsub(x[d], dc(f->origin[d])));
+        const double fl = floor(s.v);
+        w[d] = dsub(s, dc(fl));
+        long i = (long)fl % n[d];
+        if (i < 0) i += n[d];
+        i0[d] = i;
+        i1[d] = (i + 1 == n[d]) ? 0 : i + 1;   /* periodic grids only */
+    }
+    for (int c = 0; c < 3; ++c) {
+        dual6 e[4];
+        for (int q = 0; q < 4; ++q) {
+            const long j = (q & 1) ? i1[1] : i0[1], k = (q >> 1) ? i1[2] : i0[2];
+            e[q] = dlerp(dc(grid_node(f, i0[0], j, k, c)), dc(grid_node(f, i1[0], j, k, c)), w[0]);
+        }
+        out[c] = dlerp(dlerp(e[0], e[1], w[1]), dlerp(e[2], e[3], w[1]), w[2]);
+    }
+}
+
+static void dboris(const dual6 v[3], const dual6 B[3], double h, dual6 out[3])
+{
+    dual6 t[3], s[3], vt[3], vp[3], vps[3];
+    for (int c = 0; c < 3; ++c) t[c] = dscale(h, B[c]);
+    const dual6 den = dadd(dc(1.0), dadd(dadd(dmul(t[0], t[0]), dmul(t[1], t[1])), dmul(t[2], t[2])));
+    for (int c = 0; c < 3; ++c) s[c] = ddiv(dscale(2.0, t[c]), den);
+    dcross(v, t, vt);
+    for (int c = 0; c < 3; ++c) vp[c] = dadd(v[c], vt[c]);
+    dcross(vp, s, vps);
+    for (int c = 0; c < 3; ++c) out[c] = dadd(v[c], vps[c]);
+}
+
+int orc_boris_tangent(const orc_desc* d, const orc_field* B, const double xv0[6], double xT[3],
+                      double J[18])
+{
+    if (B->kind != ORC_FIELD_GRID || B->extrap != ORC_EXTRAP_PERIODIC) return ORC_ERR_ARG;
+    long nt, nout;
+    const int e = orc_prepare_checks(d, &nt, &nout);
+    if (e != ORC_OK) return e;
+    dual6 x[3], v[3], Bv[3], vn[3];
+    for (int c = 0; c < 3; ++c) {
+        x[c] = dc(xv0[c]);
+        x[c].d[c] = 1.0;
+        v[c] = dc(xv0[3 + c]);
+        v[c].d[3 + c] = 1.0;
+    }
+    const double dt = d->dt;
+    dgrid(B, x, Bv);
+    dboris(v, Bv, (d->q2m * 0.5) * (-0.5 * dt), vn);                 /* boris.jl:514 */
+    for (int c = 0; c < 3; ++c) v[c] = vn[c];
+    for (long it = 1; it <= nt; ++it) {                               /* boris.jl:452-476 */
+        dgrid(B, x, Bv);
+        dboris(v, Bv, (d->q2m * 0.5) * dt, vn);
+        for (int c = 0; c < 3; ++c) {
+            v[c] = vn[c];
+            x[c] = dadd(x[c], dscale(dt, v[c]));
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        xT[c] = x[c].v;
+        for (int k = 0; k < 6; ++k) J[6 * c + k] = x[c].d[k];
+    }
+    return ORC_OK;
+}
+
 int orc_solve(const orc_desc* d, const orc_field* E, const orc_field* B,
               const double* xv0, long N, double* out_u, double* out_t,
               int* out_len, int* retcode, int nthreads, double* elapsed_s)

@@ -106,6 +106,10 @@ int orc_prepare_checks(const orc_desc* d, long* nt, long* nout);
  * adaptive_boris.jl:196-213).  Records: out_u[(i*cap + k)*6 + c],
  * out_t[i*cap + k]; out_len[i] = records written (may exceed cap for the
  * adaptive solver, in which case the extra records were dropped). */
+/* d x(T) / d (x0, v0) of a fixed-step standard Boris solve with E = 0 and a
+ * periodic gridded B, by forward-mode dual numbers: xT[3], J[3][6] row-major */
+int orc_boris_tangent(const orc_desc* d, const orc_field* B, const double xv0[6], double xT[3],
+                      double J[18]);
 int orc_solve(const orc_desc* d, const orc_field* E, const orc_field* B,
               const double* xv0, long N, double* out_u, double* out_t,
               int* out_len, int* retcode, int nthreads, double* elapsed_s);

@@ -7,6 +7,7 @@
 // saved record (src/boris/boris_kernel.jl:130-236).
 #include "tpamd_device.h"
 #include "tpamd_grid.h"
+#include "tpamd_adjoint.h"
 #include "tpamd.h"
 
 #include <hip/hip_runtime.h>
@@ -19,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace tpamd;
@@ -483,6 +485,226 @@ struct S32G : S32C {
         return (double)sqrtf(dot3f(B, B));
     }
 };
+
+// ---------------------------------------------------------------------------
+// Adjoint (see tpamd_adjoint.h).  The forward pass reuses the specialised
+// Boris policies, so its trajectory is bit-identical to a solve of the same
+// problem; FP64 checkpoints are taken every K steps.
+// ---------------------------------------------------------------------------
+template <bool F32, bool PER>
+using AdjState = typename std::conditional<F32, S32G<kEUniform, 0, PER, false>,
+                                           S64G<kEUniform, 0, PER, double, false>>::type;
+
+template <bool F32, bool PER>
+__device__ __forceinline__ void adj_ckpt_store(const AdjParams& a, long long N, int seg, long long i,
+                                               const AdjState<F32, PER>& s)
+{
+    double* c = a.ckpt + (size_t)seg * 9 * N + i;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if constexpr (F32) {
+            c[(size_t)d * N] = (double)s.pos.hi[d] + (double)s.pos.lo[d];
+            c[(size_t)(3 + d) * N] = (double)s.pos.wrap[d];
+        } else {
+            c[(size_t)d * N] = s.pos.x[d];
+            c[(size_t)(3 + d) * N] = 0.0;
+        }
+        c[(size_t)(6 + d) * N] = (double)s.v[d];
+    }
+}
+
+template <bool F32, bool PER>
+__device__ __forceinline__ void adj_ckpt_load(const AdjParams& a, long long N, int seg, long long i,
+                                              AdjState<F32, PER>& s)
+{
+    const double* c = a.ckpt + (size_t)seg * 9 * N + i;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double r = c[(size_t)d * N];
+        if constexpr (F32) {
+            s.pos.hi[d] = (float)r;
+            s.pos.lo[d] = (float)(r - (double)s.pos.hi[d]);
+            s.pos.wrap[d] = (int)c[(size_t)(3 + d) * N];
+        } else {
+            s.pos.x[d] = r;
+        }
+        s.v[d] = (typename AdjState<F32, PER>::R)c[(size_t)(6 + d) * N];
+    }
+}
+
+// cell coordinates of the current position, as the forward gather computes them
+template <bool F32, bool PER>
+__device__ __forceinline__ void adj_cellcoord(const KParams& p, const AdjState<F32, PER>& s,
+                                              typename AdjState<F32, PER>::R sc[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if constexpr (F32) sc[d] = fmaf(s.pos.hi[d], p.B.inv_hf[d], s.pos.lo[d] * p.B.inv_hf[d]);
+        else sc[d] = (s.pos.x[d] - p.B.origin[d]) * p.B.inv_h[d];
+    }
+}
+
+// one reverse step at the state held in s: (xbar, ubar) <- VJP of the step
+// that started there, with rotation factor h and position step dt
+template <bool F32, bool PER>
+__device__ __forceinline__ void adj_step_back(const KParams& p, const AdjParams& a,
+                                              const AdjState<F32, PER>& s, const typename AdjState<F32, PER>::R u[3],
+                                              double h, double dt, typename AdjState<F32, PER>::R xb[3],
+                                              typename AdjState<F32, PER>::R ub[3], bool with_dt)
+{
+    using R = typename AdjState<F32, PER>::R;
+    R sc[3], w[3], c[8][3], B[3];
+    int i0[3], i1[3];
+    unsigned node[8];
+    adj_cellcoord<F32, PER>(p, s, sc);
+    if (!adj_locate<PER, R>(p.B, sc, i0, i1, w)) return;
+    if (p.B.dtype == 1) adj_corners<float, R>(p.B, i0, i1, c, node);
+    else adj_corners<double, R>(p.B, i0, i1, c, node);
+    adj_value<R>(c, w, B);
+    const R hh = (R)h;
+    R t[3], g[3], tb[3], un[3], Bb[3], xg[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t[k] = hh * B[k];
+        g[k] = with_dt ? ub[k] + (R)dt * xb[k] : ub[k];
+    }
+    boris_vjp<R>(u, t, g, un, tb);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Bb[k] = hh * tb[k];
+    adj_scatter<R>(a, p.B, c, node, w, Bb, xg);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xb[k] += xg[k];
+        ub[k] = un[k];
+    }
+}
+
+template <bool F32, bool PER>
+__global__ void __launch_bounds__(256) k_adj_forward(const KParams p, const AdjParams a)
+{
+    using S = AdjState<F32, PER>;
+    using R = typename S::R;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double lsum = 0.0;
+    if (i < p.N) {
+        S s;
+        s.load(p, i);
+        const double dt = p.dt;
+        s.set_dt(dt);
+        R vt[3];
+        s.template U<0>(p, s.v, -0.5 * dt, p.t0, vt);            // boris.jl:514
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+        double t = fma(-0.5, dt, p.t0);
+        for (long long n = 0; n < p.nt; ++n) {
+            if (n % a.K == 0) adj_ckpt_store<F32, PER>(a, p.N, (int)(n / a.K), i, s);
+            t += dt;
+            s.template U<0>(p, s.v, dt, t, vt);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+            typename S::Pos xn;
+            do_step(p, s, s.v, dt, xn);
+            s.pos = xn;
+        }
+        double x[3];
+        s.abs_pos(p, s.pos, x);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a.xfinal[3 * i + c] = x[c];
+            const double d = x[c] - a.targets[3 * i + c];
+            lsum += d * d;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
+    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(a.loss, lsum / (double)p.N);
+}
+
+template <bool F32, bool PER>
+__global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjParams a)
+{
+    using S = AdjState<F32, PER>;
+    using R = typename S::R;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.N) return;
+    const long long N = p.N;
+    const double dt = p.dt;
+    const double h = (p.q2m * 0.5) * dt;
+    R xb[3], ub[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        xb[c] = (R)(2.0 / (double)N * (a.xfinal[3 * i + c] - a.targets[3 * i + c]));
+        ub[c] = (R)0;
+    }
+    constexpr int NC = F32 ? 9 : 6;
+    R* buf = (R*)a.seg;
+    for (int seg = a.nseg - 1; seg >= 0; --seg) {
+        const long long n0 = (long long)seg * a.K;
+        const int len = (int)((p.nt - n0) < a.K ? (p.nt - n0) : a.K);
+        S s;
+        adj_ckpt_load<F32, PER>(a, N, seg, i, s);
+        s.set_dt(dt);
+        R vt[3];
+        for (int j = 0; j < len; ++j) {                       // recompute the segment
+            R* b = buf + (size_t)j * NC * N + i;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if constexpr (F32) {
+                    b[(size_t)d * N] = s.pos.hi[d];
+                    b[(size_t)(3 + d) * N] = s.pos.lo[d];
+                    b[(size_t)(6 + d) * N] = s.v[d];
+                } else {
+                    b[(size_t)d * N] = s.pos.x[d];
+                    b[(size_t)(3 + d) * N] = s.v[d];
+                }
+            }
+            s.template U<0>(p, s.v, dt, 0.0, vt);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+            typename S::Pos xn;
+            do_step(p, s, s.v, dt, xn);
+            s.pos = xn;
+        }
+        for (int j = len - 1; j >= 0; --j) {                  // and walk it backwards
+            const R* b = buf + (size_t)j * NC * N + i;
+            R u[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if constexpr (F32) {
+                    s.pos.hi[d] = b[(size_t)d * N];
+                    s.pos.lo[d] = b[(size_t)(3 + d) * N];
+                    u[d] = b[(size_t)(6 + d) * N];
+                } else {
+                    s.pos.x[d] = b[(size_t)d * N];
+                    u[d] = b[(size_t)(3 + d) * N];
+                }
+            }
+            adj_step_back<F32, PER>(p, a, s, u, h, dt, xb, ub, true);
+        }
+    }
+    // the initial back half-step u(-1/2) = U(v0, x0, -dt/2) (boris.jl:514)
+    S s0;
+    s0.load(p, i);
+    R v0[3] = {s0.v[0], s0.v[1], s0.v[2]};
+    adj_step_back<F32, PER>(p, a, s0, v0, (p.q2m * 0.5) * (-0.5 * dt), 0.0, xb, ub, false);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a.grad_xv0[6 * i + c] = (double)xb[c];
+        a.grad_xv0[6 * i + 3 + c] = (double)ub[c];
+    }
+}
+
+__global__ void k_zero_f32(float* p, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+__global__ void k_zero_f64(double* p, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
 
 // ---------------------------------------------------------------------------
 // Batched ensembles: a persistent grid of waves pulls chunks of particles from
@@ -1459,6 +1681,123 @@ int32_t tpamd_solve(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_fi
     if (desc->adaptive && ml > cap)
         return fail(ctx, TPAMD_ERR_CAPACITY, "adaptive solve produced %lld records for a capacity of %lld",
                     (long long)ml, (long long)cap);
+    return TPAMD_OK;
+}
+
+int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* B,
+                      const double* xv0, int64_t N, const double* targets, int32_t ckpt_interval,
+                      double* loss, double* dL_dB, double* dL_dxv0, tpamd_stats* stats)
+{
+    if (!ctx) return fail(nullptr, TPAMD_ERR_ARGUMENT, "null context");
+    if (!desc || !B || !loss) return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
+    if (N < 1 || !xv0 || !targets || !dL_dB || !dL_dxv0)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "adjoint needs N >= 1 and all buffers");
+    if (desc->alg != TPAMD_ALG_BORIS || desc->adaptive)
+        return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the adjoint covers the fixed-step standard Boris pusher");
+    if (B->d.kind != kFieldGrid) return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the adjoint needs a gridded B");
+    if (desc->bnd_flags) return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the adjoint takes no boundary");
+    if (desc->dt_pp) return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the adjoint takes one dt");
+    if (ckpt_interval < 1) return fail(ctx, TPAMD_ERR_ARGUMENT, "checkpoint interval must be >= 1");
+    tpamd_field zero;
+    std::memset(&zero, 0, sizeof zero);
+    zero.d.kind = kFieldZero;
+    tpamd_solve_desc d = *desc;
+    d.save_start = 0;
+    d.save_end = 1;
+    d.save_everystep = 0;
+    d.nout_cap = 1;
+    KParams p;
+    int64_t nt = 0, nout = 0;
+    int rc = fill_params(ctx, &d, &zero, B, nullptr, N, nullptr, nullptr, nullptr, nullptr, p, nt, nout);
+    if (rc) return rc;
+    if (nt < 1) return fail(ctx, TPAMD_ERR_ARGUMENT, "the adjoint needs at least one step");
+    const int K = ckpt_interval;
+    const int nseg = (int)((nt + K - 1) / K);
+    const bool f32 = desc->precision == TPAMD_FP32C;
+    const int nc = f32 ? 9 : 6;
+    const size_t rs = f32 ? 4 : 8;
+    const size_t nnodes = (size_t)B->d.n[0] * B->d.n[1] * B->d.n[2];
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t b_xv = al(sizeof(double) * 6 * N), b_tg = al(sizeof(double) * 3 * N),
+                 b_ck = al(sizeof(double) * 9 * (size_t)nseg * N), b_sg = al(rs * nc * (size_t)K * N),
+                 b_gb = al(sizeof(float) * 3 * nnodes), b_gx = al(sizeof(double) * 6 * N),
+                 b_ls = al(sizeof(double)), b_xf = al(sizeof(double) * 3 * N);
+    const size_t total = b_xv + b_tg + b_ck + b_sg + b_gb + b_gx + b_ls + b_xf;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    char* buf = nullptr;
+    HIP_TRY(ctx, hipMalloc((void**)&buf, total));
+    char* q = buf;
+    double* d_xv = (double*)q; q += b_xv;
+    double* d_tg = (double*)q; q += b_tg;
+    AdjParams a;
+    a.ckpt = (double*)q; q += b_ck;
+    a.seg = (void*)q; q += b_sg;
+    a.gradB = (float*)q; q += b_gb;
+    a.grad_xv0 = (double*)q; q += b_gx;
+    a.loss = (double*)q; q += b_ls;
+    a.xfinal = (double*)q;
+    a.targets = d_tg;
+    a.K = K;
+    a.nseg = nseg;
+    p.xv0 = d_xv;
+    hipStream_t st = ctx->stream;
+    const bool per = B->d.extrap == kExtrapPeriodic;
+    const dim3 block(256), grid((unsigned)((N + 255) / 256));
+    const long long ng = 3 * (long long)nnodes;
+    float ms_f = 0.f, ms_r = 0.f;
+    hipError_t e = hipMemcpyAsync(d_xv, xv0, sizeof(double) * 6 * N, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_tg, targets, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, a.gradB, ng);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, a.loss, 1LL);
+        e = hipEventRecord(ctx->ev0, st);
+    }
+    if (e == hipSuccess) {
+        if (f32) {
+            if (per) hipLaunchKernelGGL((k_adj_forward<true, true>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_forward<true, false>), grid, block, 0, st, p, a);
+        } else {
+            if (per) hipLaunchKernelGGL((k_adj_forward<false, true>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_forward<false, false>), grid, block, 0, st, p, a);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms_f, ctx->ev0, ctx->ev1);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, st);
+    if (e == hipSuccess) {
+        if (f32) {
+            if (per) hipLaunchKernelGGL((k_adj_reverse<true, true>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_reverse<true, false>), grid, block, 0, st, p, a);
+        } else {
+            if (per) hipLaunchKernelGGL((k_adj_reverse<false, true>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_reverse<false, false>), grid, block, 0, st, p, a);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms_r, ctx->ev0, ctx->ev1);
+    std::vector<float> gb;
+    if (e == hipSuccess) {
+        gb.resize(3 * nnodes);
+        e = hipMemcpy(gb.data(), a.gradB, sizeof(float) * 3 * nnodes, hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess) e = hipMemcpy(dL_dxv0, a.grad_xv0, sizeof(double) * 6 * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(loss, a.loss, sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(ctx, TPAMD_ERR_DEVICE, "adjoint failed: %s", hipGetErrorString(e));
+    for (size_t j = 0; j < 3 * nnodes; ++j) dL_dB[j] = (double)gb[j];
+    if (stats) {
+        stats->kernel_ms = ms_f;
+        stats->aux_ms = ms_r;
+        stats->particle_steps = nt * N;
+        stats->nt = nt;
+        stats->nout = 1;
+        stats->max_len = 1;
+        stats->total_ms = ms_f + ms_r;
+    }
     return TPAMD_OK;
 }
 

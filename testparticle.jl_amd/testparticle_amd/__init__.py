@@ -23,5 +23,6 @@ from .problem import (Boundary, EnsembleContext, InsideSphere, OutsideBox, Outsi
                       OutsideNaN, OutsideSphere, TerminateOutside, TraceProblem, remake)
 from .solution import EnsembleSolution, TraceSolution
 from .solve import initial_states, make_desc, plan, solve, solve_batch
+from .adjoint import AdjointResult, loss_gradient, trajectory_loss
 
 __all__ = [n for n in dir() if not n.startswith("_")]

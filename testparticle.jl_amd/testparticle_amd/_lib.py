@@ -63,6 +63,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("particle_steps", C.c_int64),
         ("nt", C.c_int64), ("nout", C.c_int64), ("max_len", C.c_int64),
+        ("aux_ms", C.c_double),
     ]
 
 
@@ -71,13 +72,29 @@ EXPORTS = [
     "tpamd_field_zero", "tpamd_field_uniform", "tpamd_field_affine", "tpamd_field_dipole",
     "tpamd_field_grid", "tpamd_field_grid_cells", "tpamd_field_uxb", "tpamd_field_eval",
     "tpamd_field_free",
-    "tpamd_plan", "tpamd_solve", "tpamd_solve_device",
+    "tpamd_plan", "tpamd_solve", "tpamd_solve_device", "tpamd_adjoint",
     "tpamd_batch_create", "tpamd_batch_autotune", "tpamd_batch_run", "tpamd_batch_free",
     "tpamd_solve_batch",
 ]
 
 _lib = None
 _lock = threading.Lock()
+
+
+def _bind_one_hip_runtime():
+    """PyTorch-ROCm wheels ship their own libamdhip64 whose soname
+    (libamdhip64.so.7) equals the system one the library links against, but
+    torch requests it by a different file name.  Loaded first, torch's copy
+    satisfies the library's dependency and the process holds ONE HIP runtime;
+    loaded after the library, torch would bring a second HIP/HSA runtime that
+    finds no device.  So torch, when installed, is imported before the
+    library is opened.  Set TPAMD_NO_TORCH=1 to skip this."""
+    if os.environ.get("TPAMD_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def load_library(path: str = LIB_PATH):
@@ -90,6 +107,7 @@ def load_library(path: str = LIB_PATH):
             raise RuntimeError(
                 f"HIP library {path} is missing: build it with "
                 "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+        _bind_one_hip_runtime()
         L = C.CDLL(path)
         vp, d3, i64 = C.c_void_p, C.POINTER(C.c_double), C.c_int64
         pp = C.POINTER(C.c_void_p)
@@ -106,6 +124,8 @@ def load_library(path: str = LIB_PATH):
                                        C.c_int32, pp]
         L.tpamd_field_uxb.argtypes = [vp, d3, vp, pp]
         L.tpamd_field_grid_cells.argtypes = [vp, vp, C.c_int32]
+        L.tpamd_adjoint.argtypes = [vp, C.POINTER(SolveDesc), vp, d3, i64, d3, C.c_int32, d3,
+                                    d3, d3, C.POINTER(Stats)]
         L.tpamd_field_eval.argtypes = [vp, vp, vp, d3, i64, C.c_double, C.c_int32, d3]
         L.tpamd_field_free.argtypes = [vp, vp]
         L.tpamd_plan.argtypes = [vp, C.POINTER(SolveDesc), C.POINTER(i64), C.POINTER(i64)]

@@ -1,0 +1,95 @@
+"""Adjoint (reverse-mode) gradients of the trajectory loss
+L = (1/N) sum |x_i(T) - x*_i|^2 through the fixed-step Boris pusher, checked
+against forward-mode dual numbers through the CPU oracle (dL/d(x0, v0)) and
+against central finite differences of the FP64 device solve (dL/dB)."""
+import numpy as np
+import pytest
+
+import testparticle_amd as TP
+from testparticle_amd import workloads as W
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L_BOX = 1.0e7
+
+
+def _setup(n=16, N=256, nsteps=128, seed=5, dB=0.3):
+    A = W.turbulent_field(n, L_BOX, 5e-9, dB, seed, dtype=np.float64)
+    ax = W.grid_axes(n, L_BOX)
+    rng = np.random.default_rng(seed + 1)
+    x = rng.uniform(0, L_BOX, size=(N, 3))
+    v = W.isotropic_directions(rng, N) * np.sqrt(2 * 1.0e4 * TP.qi / TP.mi)
+    xv0 = np.hstack([x, v])
+    Tg = 2 * np.pi * TP.mi / (TP.qi * 5e-9)
+    dt = Tg / 40
+    targets = rng.uniform(0, L_BOX, size=(N, 3))
+    return A, ax, xv0, dt, (0.0, nsteps * dt), targets
+
+
+def _prob(A, ax, xv0, tspan):
+    B = TP.GridField(A, *ax, bc=TP.PeriodicBC())
+    return TP.TraceProblem(xv0, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField())), B
+
+
+def _loss_fp64(A, ax, xv0, tspan, dt, targets):
+    prob, _ = _prob(A, ax, xv0, tspan)
+    xT = TP.solve(prob, TP.Boris(), dt=dt, save_everystep=False).final_states()[:, :3]
+    return float(np.mean(np.sum((xT - targets) ** 2, axis=1)))
+
+
+def test_forward_loss_matches_solver():
+    A, ax, xv0, dt, tspan, tg = _setup()
+    prob, _ = _prob(A, ax, xv0, tspan)
+    r = TP.loss_gradient(prob, dt=dt, targets=tg, precision="fp64", ckpt_interval=32)
+    assert r.loss == pytest.approx(_loss_fp64(A, ax, xv0, tspan, dt, tg), rel=1e-13)
+
+
+@pytest.mark.parametrize("precision,rtol", [("fp64", 1e-8), ("fp32c", 1e-4)])
+def test_initial_state_gradient_vs_oracle_duals(precision, rtol):
+    A, ax, xv0, dt, tspan, tg = _setup(N=64)
+    prob, B = _prob(A, ax, xv0, tspan)
+    r = TP.loss_gradient(prob, dt=dt, targets=tg, precision=precision, ckpt_interval=16)
+    Bo = O.grid(B.data, B.origin, B.spacing, O.EXTRAP_PERIODIC)
+    N = xv0.shape[0]
+    ref = np.zeros((N, 6))
+    for i in range(N):
+        xT, J = O.boris_tangent(xv0[i], Bo, q2m=TP.qi / TP.mi, tspan=tspan, dt=dt)
+        ref[i] = 2.0 / N * (xT - tg[i]) @ J
+    scale = np.abs(ref).max(axis=0)          # per-component scale (positions vs velocities)
+    err = np.abs(r.dL_dxv0 - ref) / scale
+    print(f"{precision}: max error of dL/d(x0,v0) relative to the component scale {err.max():.2e}")
+    assert err.max() < rtol
+
+
+@pytest.mark.parametrize("precision,rtol", [("fp64", 1e-5), ("fp32c", 1e-4)])
+def test_grid_gradient_vs_central_differences(precision, rtol):
+    A, ax, xv0, dt, tspan, tg = _setup(n=12, N=2048, nsteps=96)
+    prob, _ = _prob(A, ax, xv0, tspan)
+    r = TP.loss_gradient(prob, dt=dt, targets=tg, precision=precision, ckpt_interval=24)
+    rng = np.random.default_rng(11)
+    g = np.abs(r.dL_dB)
+    # 32 random grid values among those the particles actually touch
+    cand = np.argwhere(g > 1e-3 * g.max())
+    picks = cand[rng.choice(len(cand), 32, replace=False)]
+    h = 1e-6 * 5e-9
+    worst = 0.0
+    for c, i, j, k in picks:
+        Ap, Am = A.copy(), A.copy()
+        Ap[c, i, j, k] += h
+        Am[c, i, j, k] -= h
+        fd = (_loss_fp64(Ap, ax, xv0, tspan, dt, tg) - _loss_fp64(Am, ax, xv0, tspan, dt, tg)) / (2 * h)
+        worst = max(worst, abs(r.dL_dB[c, i, j, k] - fd) / abs(fd))
+    print(f"{precision}: worst relative error of dL/dB vs central FD on 32 nodes {worst:.2e}")
+    assert worst < rtol
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    A, ax, xv0, dt, tspan, tg = _setup(N=512, nsteps=100)
+    prob, _ = _prob(A, ax, xv0, tspan)
+    rs = [TP.loss_gradient(prob, dt=dt, targets=tg, precision="fp64", ckpt_interval=K)
+          for K in (1, 7, 32, 100, 1000)]
+    for r in rs[1:]:
+        np.testing.assert_array_equal(r.dL_dxv0, rs[0].dL_dxv0)
+        np.testing.assert_allclose(r.dL_dB, rs[0].dL_dB, rtol=1e-5, atol=1e-7 * np.abs(rs[0].dL_dB).max())

@@ -236,3 +236,40 @@ def test_adaptive_uniform_matches_fixed(orc):
     np.testing.assert_allclose(ta[:100], tf[:100], rtol=1e-12, atol=1e-14)
     np.testing.assert_allclose(ua[:100], uf[:100], rtol=1e-9, atol=1e-11)
     assert ta[-1] == pytest.approx(123.4 * dt, rel=1e-14)
+
+
+def test_boris_tangent_matches_solve_and_finite_differences(orc):
+    # forward-mode duals through the fixed-step Boris solve: the primal is the
+    # solver's own final position, dx(T)/dx0 is the identity in a uniform B,
+    # and in a periodic gridded B the Jacobian matches central differences
+    q2m, dt, nt = 1.0, 0.05, 200
+    u0 = np.array([0.3, -0.2, 0.1, 0.8, -0.4, 0.3])
+    const = orc.grid(np.broadcast_to([0.1, -0.3, 1.2], (3, 3, 3, 3)).copy(), (0.0, 0.0, 0.0),
+                     (1.0, 1.0, 1.0), orc.EXTRAP_PERIODIC)
+    xT, J = orc.boris_tangent(u0, const, q2m=q2m, tspan=(0.0, nt * dt), dt=dt)
+    r = orc.solve(u0, orc.zero(), const, q2m=q2m, tspan=(0.0, nt * dt), dt=dt,
+                  save_everystep=False)
+    np.testing.assert_array_equal(xT, r.traj(0)[1][-1][:3])
+    np.testing.assert_allclose(J[:, :3], np.eye(3), atol=1e-13)
+
+    n, L = (6, 5, 7), 4.0
+    h = tuple(L / n[d] for d in range(3))
+    data = RNG.normal(scale=0.4, size=(n[2], n[1], n[0], 3)) + np.array([0.0, 0.0, 1.0])
+    g = orc.grid(data, (0.0, 0.0, 0.0), h, orc.EXTRAP_PERIODIC)
+    xT, J = orc.boris_tangent(u0, g, q2m=q2m, tspan=(0.0, nt * dt), dt=dt)
+    r = orc.solve(u0, orc.zero(), g, q2m=q2m, tspan=(0.0, nt * dt), dt=dt, save_everystep=False)
+    np.testing.assert_array_equal(xT, r.traj(0)[1][-1][:3])
+    eps = 1e-6
+    fd = np.zeros((3, 6))
+    for j in range(6):
+        up, um = u0.copy(), u0.copy()
+        up[j] += eps
+        um[j] -= eps
+        xp = orc.solve(up, orc.zero(), g, q2m=q2m, tspan=(0.0, nt * dt), dt=dt,
+                       save_everystep=False).traj(0)[1][-1][:3]
+        xm = orc.solve(um, orc.zero(), g, q2m=q2m, tspan=(0.0, nt * dt), dt=dt,
+                       save_everystep=False).traj(0)[1][-1][:3]
+        fd[:, j] = (xp - xm) / (2 * eps)
+    np.testing.assert_allclose(J, fd, rtol=0, atol=1e-6 * np.abs(fd).max())
+    # a dropped field-gradient term would make the position block the identity
+    assert np.abs(J[:, :3] - np.eye(3)).max() > 1e-3
