@@ -1,16 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark: gridded-field ensemble Boris on one MI355X.
+"""Benchmark: ensemble Boris pushers on one MI355X.
 
-Workload (default, the gridded headline): 256^3 periodic turbulent box of side
-1e7 m (B = B0 z + k^-5/3 random-phase dB, dB_rms = 0.5 B0, B0 = 5 nT),
-E = -u x B with u = (4e5, 0, 0) m/s derived in-kernel, 4,194,304 protons
-(1-100 keV, isotropic), MultistepBoris4(n=2), dt = T_g(B0)/40, 5,000 steps,
-start and end records saved.
+Headline (`value`): gridded turbulent box, 256^3 periodic grid of side 1e7 m
+(B = B0 z + k^-5/3 random-phase dB, dB_rms = 0.5 B0, B0 = 5 nT), E = -u x B
+with u = (4e5, 0, 0) m/s derived in-kernel, 4,194,304 protons (1-100 keV,
+isotropic), MultistepBoris4(n=2), dt = T_g(B0)/40, 5,000 steps, start and end
+records saved.  One "step" of this script = one full solve (all particles,
+all 5,000 Boris steps) with initial states and records resident in HBM.
+`value` is the FP32-compensated pusher's particle-steps/s; the FP64 pusher is
+timed the same way beside it, and the FP32C final positions are compared with
+FP64 at several horizons next to the FP64 chaos floor (FP64 against FP64
+started one ulp away).
 
-One "step" of this script = one full solve (all particles, all 5,000 Boris
-steps) with initial states and records resident in HBM.  `value` is the
-FP32-compensated pusher's particle-steps/s; the FP64 pusher is timed the same
-way beside it, and the FP32C result's error against FP64 is reported.
+Secondary sections (same JSON line, key "secondary"):
+  dipole   analytic Earth dipole, 1,048,576 protons, standard Boris, per-particle
+           dt = T_g/32, 10,000 steps, removal inside 1 R_E (FP64 and FP32C)
+  batch    64 uneven turbulent-box instances (32^3..128^3, ~2e7 protons),
+           AdaptiveBoris(0.1), exits at the box faces, <= 2,000 steps, one
+           persistent graph-captured launch after autotuning (FP32C and FP64)
+  adjoint  64^3 grid, dB = 0.3 B0, 262,144 protons at 10 keV, standard Boris,
+           2,048 steps, checkpoints every 64: gradient cost against the plain
+           forward solve and dL/dB against central differences of the FP64 solve
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -34,6 +44,10 @@ FP32_PEAK = 157.3e12       # FLOP/s, vector FP32 (packed)
 # algorithmic bytes per particle-step of the gather: one vector field,
 # 8 nodes x 3 components (E = -u x B derived in-kernel)
 GATHER_BYTES = {"f32": 8 * 3 * 4, "f64": 8 * 3 * 8}
+# algorithmic FLOPs per particle-step (FMA = 2; div, sqrt = 1): standard Boris
+# with E = 0 (37) + position update (6) + dipole evaluation (27)
+DIPOLE_FLOP = 37 + 6 + 27
+SECTIONS = ("headline", "dipole", "batch", "adjoint")
 
 
 def log(*a):
@@ -48,18 +62,28 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def timed_runs(ens, torch, warmup, steps):
+def timed_runs(run, torch, warmup, steps):
+    """warmup + `steps` timed calls of run() bracketed by device syncs;
+    returns (wall seconds per call, mean kernel ms, last stats)."""
     for _ in range(warmup):
-        ens.run()
+        run()
     torch.cuda.synchronize()
     kms = []
     t0 = time.perf_counter()
+    st = None
     for _ in range(steps):
-        st = ens.run()
+        st = run()
         kms.append(st.kernel_ms)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
     return wall, float(np.mean(kms)), st
+
+
+def rel_pos_err(xa, xb):
+    d = np.linalg.norm(xa - xb, axis=1) / np.linalg.norm(xb, axis=1)
+    d = d[np.isfinite(d)]
+    return {"median": float(np.median(d)), "p95": float(np.percentile(d, 95)),
+            "max": float(np.max(d))}
 
 
 def cpu_baseline(w, A64, ax, nthreads, target_s=12.0):
@@ -84,38 +108,53 @@ def cpu_baseline(w, A64, ax, nthreads, target_s=12.0):
                       f"MultistepBoris4(n=2) reference form, {r.elapsed:.1f} s wall"}
 
 
-def bench_turbulent(args):
-    import torch
-
-    import testparticle_amd as TP
-    from testparticle_amd import workloads as W
-    from testparticle_amd.device import DeviceEnsemble
-
+# ---------------------------------------------------------------------------
+# headline: gridded turbulent box
+# ---------------------------------------------------------------------------
+def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
     N, n, nsteps = args.particles, args.grid, args.nsteps
     log(f"[bench] generating {n}^3 turbulent field and {N} protons")
     w, A64 = W.turbulent_box(N=N, n=n, nsteps=nsteps, dtype=np.float64)
     ax = W.grid_axes(n, w.extra["L"])
     alg = TP.MultistepBoris(4, n=2)
+    horizons = [h for h in (10, 100, 1000) if h < nsteps] + [nsteps]
+    x0p = w.xv0.copy()
+    x0p[:, :3] = np.nextafter(x0p[:, :3], np.inf)        # one ulp away, for the chaos floor
     res = {}
     for prec, storage in (("fp32c", "f32"), ("fp64", "f64")):
         A = A64 if storage == "f64" else A64.astype(np.float32)
         B = TP.GridField(A, *ax, bc=TP.PeriodicBC(), layout=args.layout)
         E = TP.UxBField(w.extra["u"], B)
-        prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, E, B, TP.ZeroField()))
+        param = (w.q2m, TP.mi, E, B, TP.ZeroField())
+        prob = TP.TraceProblem(w.xv0, w.tspan, param)
         log(f"[bench] {prec}: uploading and warming up")
         ens = DeviceEnsemble(prob, alg, w.xv0, dt=w.dt, save_everystep=False, precision=prec)
-        wall, kms, st = timed_runs(ens, torch, args.warmup, args.steps)
+        wall, kms, st = timed_runs(ens.run, torch, args.warmup, args.steps)
         assert st.particle_steps == N * nsteps, st.particle_steps
-        res[prec] = dict(wall=wall, kernel_ms=kms, final=ens.final_states(), storage=storage)
+        r = dict(wall=wall, kernel_ms=kms, storage=storage, final={nsteps: ens.final_states()[:, :3]})
         log(f"[bench] {prec}: {wall * 1e3:.1f} ms/solve, kernel {kms:.1f} ms, "
             f"{N * nsteps / wall:.3e} particle-steps/s")
-        del ens, B, E, prob
+        del ens
+        if prec == "fp32c" and args.accuracy:
+            # the host-buffer entry point: PCIe copies in and out included
+            t0 = time.perf_counter()
+            TP.solve(prob, alg, dt=w.dt, save_everystep=False, precision=prec)
+            r["host_wall"] = time.perf_counter() - t0
+        if args.accuracy:
+            for h in horizons:
+                starts = [("x0", w.xv0)] + ([("x0p", x0p)] if prec == "fp64" else [])
+                for tag, xv in starts:
+                    if h == nsteps and tag == "x0":
+                        continue
+                    pr = TP.TraceProblem(xv, (0.0, h * w.dt), param)
+                    e = DeviceEnsemble(pr, alg, xv, dt=w.dt, save_everystep=False, precision=prec)
+                    e.run()
+                    key = "final" if tag == "x0" else "final_p"
+                    r.setdefault(key, {})[h] = e.final_states()[:, :3]
+                    del e
+        res[prec] = r
+        del B, E, prob
         torch.cuda.empty_cache()
-    x0 = w.xv0[:, :3]
-    x64, x32 = res["fp64"]["final"][:, :3], res["fp32c"]["final"][:, :3]
-    d = np.linalg.norm(x32 - x64, axis=1)
-    err_disp = d / np.maximum(np.linalg.norm(x64 - x0, axis=1), 1.0)
-    err_pos = d / np.linalg.norm(x64, axis=1)
     ps = N * nsteps
     r32 = res["fp32c"]
     value = ps / r32["wall"]
@@ -140,16 +179,29 @@ def bench_turbulent(args):
                    "parallelism": "single GPU, one thread per particle"},
         "fp64_value": ps / res["fp64"]["wall"],
         "fp32c_over_fp64": res["fp64"]["wall"] / r32["wall"],
-        "fp32c_err_vs_fp64": {"metric": "|x32 - x64| / |x64 - x0| at the final step",
-                              "median": float(np.median(err_disp)),
-                              "p95": float(np.percentile(err_disp, 95)),
-                              "max": float(np.max(err_disp)),
-                              "median_rel_position": float(np.median(err_pos))},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_fixed<S32G<UxB, n=2, periodic, cells>>", "kernel_ms": r32["kernel_ms"],
-                     "algorithmic_bytes_per_particle_step": GATHER_BYTES["f32"]},
+        "kernel_ms": {"fp32c": r32["kernel_ms"], "fp64": res["fp64"]["kernel_ms"]},
     }
+    if "host_wall" in r32:
+        out["fp32c_host_buffers"] = {"value": ps / r32["host_wall"], "ms_per_solve": r32["host_wall"] * 1e3,
+                                     "note": "tpamd_solve from host arrays: PCIe copies and host "
+                                             "output allocation included"}
+    if args.accuracy:
+        acc = {}
+        x64 = res["fp64"]["final"]
+        for h in horizons:
+            acc[str(h)] = {"fp32c_vs_fp64": rel_pos_err(r32["final"][h], x64[h]),
+                           "fp64_one_ulp_floor": rel_pos_err(res["fp64"]["final_p"][h], x64[h])}
+        out["fp32c_error"] = {
+            "metric": "|x_fp32c - x_fp64| / |x_fp64| of the final positions, per horizon (steps)",
+            "chaos_floor": "fp64_one_ulp_floor = FP64 started one ulp away in position vs FP64",
+            "horizons": acc}
+    out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                       "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                       "kernel": "k_fixed<S32G<UxB, n=2, periodic, cells>>" if args.layout == "cells"
+                       else "k_fixed<S32G<UxB, n=2, periodic, nodes>>",
+                       "kernel_ms": r32["kernel_ms"],
+                       "algorithmic_bytes_per_particle_step": GATHER_BYTES["f32"],
+                       "algorithmic_bytes_per_launch": bytes_launch}
     # HBM traffic per launch: FETCH_SIZE + WRITE_SIZE per particle-step from the
     # rocprofv3 --pmc passes recorded under profiles/ (scaled by the copy-probe
     # calibration), times this launch's particle-steps
@@ -166,6 +218,161 @@ def bench_turbulent(args):
         out["cpu_baseline"] = cpu_baseline(w, A64, ax, cpu_threads())
     else:
         out["cpu_baseline"] = None
+    return out
+
+
+# ---------------------------------------------------------------------------
+# secondary: analytic dipole (FLOP-bound)
+# ---------------------------------------------------------------------------
+def bench_dipole(args, torch, TP, W, DeviceEnsemble):
+    w = W.dipole_belt(N=args.dipole_particles, nsteps=10000)
+    log(f"[bench] dipole: {len(w.xv0)} protons x {w.nsteps} steps")
+    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), (w.q2m, TP.mi, TP.ZeroField(),
+                                               TP.DipoleField(w.extra["b0"], w.extra["R"]),
+                                               TP.ZeroField()))
+    out = {"workload": "dipole_belt_L4-7_10keV-10MeV", "particles": len(w.xv0),
+           "steps": w.nsteps, "alg": "Boris", "dt": "T_g(x0)/32 per particle",
+           "boundary": "removed when |x| < 1 R_E", "flop_per_particle_step": DIPOLE_FLOP}
+    fin = {}
+    for prec, peak in (("fp64", FP64_PEAK), ("fp32c", FP32_PEAK)):
+        ens = DeviceEnsemble(prob, TP.Boris(), w.xv0, dts=w.dts, nsteps=w.nsteps,
+                             save_everystep=False, isoutside=TP.InsideSphere(TP.Re),
+                             precision=prec)
+        wall, kms, st = timed_runs(ens.run, torch, 1, args.aux_steps)
+        u, _t, ln, ret = ens.host()
+        fin[prec] = (u[np.arange(len(ln)), np.maximum(ln - 1, 0)], ret)
+        rate = st.particle_steps / wall
+        flops = st.particle_steps * DIPOLE_FLOP / (kms * 1e-3)
+        out[prec] = {"value": rate, "unit": "particle-steps/s", "ms_per_solve": wall * 1e3,
+                     "kernel_ms": kms, "particle_steps": int(st.particle_steps),
+                     "terminated": int(np.sum(ret == 1)),
+                     "roofline": {"bound": "valu", "achieved": flops / 1e12, "peak": peak / 1e12,
+                                  "unit": "TFLOP/s", "frac": flops / peak,
+                                  "kernel": "k_fixed<S64, Boris>" if prec == "fp64"
+                                  else "k_fixed<S32C, Boris>"}}
+        log(f"[bench] dipole {prec}: {rate:.3e} particle-steps/s ({flops / 1e12:.1f} TFLOP/s)")
+        del ens
+    u64, r64 = fin["fp64"]
+    u32, r32 = fin["fp32c"]
+    both = (r64 == 0) & (r32 == 0)
+    v0 = np.sum(w.xv0[:, 3:] ** 2, axis=1)
+    for prec, (u, _r) in fin.items():
+        drift = np.abs(np.sum(u[:, 3:] ** 2, axis=1) - v0) / v0
+        out[prec]["energy_drift"] = {"median": float(np.median(drift[both])),
+                                     "max": float(np.max(drift[both]))}
+    out["fp32c_vs_fp64"] = rel_pos_err(u32[both, :3], u64[both, :3])
+    out["fp32c_over_fp64"] = out["fp32c"]["value"] / out["fp64"]["value"]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# secondary: batched uneven ensemble in one persistent graph launch
+# ---------------------------------------------------------------------------
+def bench_batch(args, torch, TP, W):
+    from testparticle_amd.device import DeviceBatch
+
+    log(f"[bench] batch: generating 64 instances, ~{args.batch_total:.2e} protons")
+    inst, tspan = W.batched_turbulence(n_inst=64, total=args.batch_total)
+    probs = [TP.TraceProblem(xv, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(),
+                                         TP.GridField(A, *axes), TP.ZeroField()))
+             for A, axes, xv in inst]
+    N = sum(len(xv) for _, _, xv in inst)
+    out = {"workload": "batched_turbulence_64", "instances": 64, "particles": N,
+           "grid_edges": sorted({int(A.shape[1]) for A, _, _ in inst}),
+           "grid_bytes_f32": int(sum(A.size * 4 for A, _, _ in inst)),
+           "alg": "AdaptiveBoris(safety=0.1)", "max_steps": 2000, "boundary": "exit at box faces"}
+    fin = {}
+    for prec in ("fp32c", "fp64"):
+        b = DeviceBatch(probs, TP.AdaptiveBoris(safety=0.1), precision=prec,
+                        isoutside=TP.OutsideGrid(), max_steps=2000, save_everystep=False)
+        b.run()
+        bs, ppt = b.autotune()
+        wall, kms, st = timed_runs(b.run, torch, 1, args.aux_steps)
+        fin[prec] = b.final_states()[:, :3]
+        rate = st.particle_steps / wall
+        achieved = st.particle_steps * GATHER_BYTES["f32" if prec == "fp32c" else "f64"] / (kms * 1e-3)
+        out[prec] = {"value": rate, "unit": "particle-steps/s", "ms_per_run": wall * 1e3,
+                     "kernel_ms": kms, "particle_steps": int(st.particle_steps),
+                     "autotuned": {"block": bs, "particles_per_lane": ppt},
+                     "roofline": {"bound": "hbm", "achieved": achieved / 1e9,
+                                  "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                                  "frac": achieved / HBM_PEAK, "kernel": "k_batch (adaptive)"}}
+        log(f"[bench] batch {prec}: {rate:.3e} particle-steps/s, block {bs}, ppt {ppt}")
+        b.close()
+        del b
+        torch.cuda.empty_cache()
+    out["fp32c_vs_fp64"] = rel_pos_err(fin["fp32c"], fin["fp64"])
+    out["fp32c_over_fp64"] = out["fp32c"]["value"] / out["fp64"]["value"]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# secondary: adjoint cost and gradient check
+# ---------------------------------------------------------------------------
+def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
+    n, L, B0, N, nsteps, K = 64, 1.0e7, 5.0e-9, args.adjoint_particles, 2048, 64
+    A = W.turbulent_field(n, L, B0, 0.3, 55, dtype=np.float64)
+    ax = W.grid_axes(n, L)
+    rng = np.random.default_rng(56)
+    x = rng.uniform(0, L, size=(N, 3))
+    v = W.isotropic_directions(rng, N) * math.sqrt(2 * 1.0e4 * TP.qi / TP.mi)
+    xv0 = np.ascontiguousarray(np.hstack([x, v]))
+    tg = rng.uniform(0, L, size=(N, 3))
+    dt = 2 * math.pi * TP.mi / (TP.qi * B0) / 40
+    tspan = (0.0, nsteps * dt)
+    log(f"[bench] adjoint: {n}^3 grid, {N} protons x {nsteps} steps, checkpoints every {K}")
+
+    def prob_of(Av, storage, layout="nodes"):
+        B = TP.GridField(Av, *ax, bc=TP.PeriodicBC(), storage=storage, layout=layout)
+        return TP.TraceProblem(xv0, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+
+    out = {"workload": "adjoint_turbulent_64^3", "particles": N, "steps": nsteps,
+           "checkpoint_interval": K, "dt": "T_g(B0)/40", "box": "1e7 m periodic",
+           "loss": "mean |x(T) - x*|^2"}
+    for prec, storage in (("fp32c", "f32"), ("fp64", "f64")):
+        # plain forward solve of the same problem, best of both layouts
+        fwd = {}
+        for layout in ("nodes", "cells"):
+            ens = DeviceEnsemble(prob_of(A, storage, layout), TP.Boris(), xv0, dt=dt,
+                                 save_everystep=False, save_start=False, precision=prec)
+            _w, kms, _st = timed_runs(ens.run, torch, 1, args.aux_steps)
+            fwd[layout] = kms
+            del ens
+        pr = prob_of(A, storage)
+        rs = [TP.loss_gradient(pr, dt=dt, targets=tg, precision=prec, ckpt_interval=K)
+              for _ in range(1 + args.aux_steps)][1:]
+        f_ms = float(np.mean([r.forward_ms for r in rs]))
+        r_ms = float(np.mean([r.reverse_ms for r in rs]))
+        best = min(fwd.values())
+        out[prec] = {"forward_solve_ms": fwd, "adjoint_forward_ms": f_ms, "adjoint_reverse_ms": r_ms,
+                     "cost_over_forward": (f_ms + r_ms) / best, "loss": rs[-1].loss}
+        out[prec]["_grad"] = rs[-1]
+        log(f"[bench] adjoint {prec}: forward solve {best:.2f} ms, adjoint {f_ms:.2f} + {r_ms:.2f} ms "
+            f"= {(f_ms + r_ms) / best:.2f}x")
+
+    # dL/dB on 32 nodes against central differences of the FP64 device solve
+    def loss64(Av):
+        xT = TP.solve(prob_of(Av, "f64"), TP.Boris(), dt=dt, save_everystep=False,
+                      save_start=False).final_states()[:, :3]
+        return float(np.mean(np.sum((xT - tg) ** 2, axis=1)))
+
+    g64 = out["fp64"]["_grad"].dL_dB
+    cand = np.argwhere(np.abs(g64) > 1e-3 * np.abs(g64).max())
+    picks = cand[np.random.default_rng(57).choice(len(cand), 32, replace=False)]
+    h = 1e-6 * B0
+    fd = []
+    for c, i, j, k in picks:
+        Ap, Am = A.copy(), A.copy()
+        Ap[c, i, j, k] += h
+        Am[c, i, j, k] -= h
+        fd.append((loss64(Ap) - loss64(Am)) / (2 * h))
+    fd = np.array(fd)
+    for prec in ("fp32c", "fp64"):
+        g = out[prec].pop("_grad").dL_dB
+        ga = np.array([g[c, i, j, k] for c, i, j, k in picks])
+        rel = np.abs(ga - fd) / np.abs(fd)
+        out[prec]["dL_dB_vs_central_fd"] = {"nodes": 32, "h": h, "median_rel": float(np.median(rel)),
+                                            "max_rel": float(np.max(rel))}
     return out
 
 
@@ -196,6 +403,14 @@ def main():
     ap.add_argument("--layout", choices=("cells", "nodes"), default="cells",
                     help="grid storage on the device (cells: one cache line per gather)")
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--no-accuracy", dest="accuracy", action="store_false",
+                    help="skip the FP32C-vs-FP64 horizon runs")
+    ap.add_argument("--sections", default=",".join(SECTIONS),
+                    help="comma-separated subset of " + ",".join(SECTIONS))
+    ap.add_argument("--aux-steps", type=int, default=2, help="timed runs per secondary section")
+    ap.add_argument("--dipole-particles", type=int, default=1 << 20)
+    ap.add_argument("--batch-total", type=int, default=20_000_000)
+    ap.add_argument("--adjoint-particles", type=int, default=1 << 18)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the ensemble pusher runs on a single GPU "
@@ -204,7 +419,27 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
-    out = bench_turbulent(args)
+    sections = [s for s in args.sections.split(",") if s]
+    bad = [s for s in sections if s not in SECTIONS]
+    if bad:
+        ap.error(f"unknown sections {bad}")
+
+    import torch
+
+    import testparticle_amd as TP
+    from testparticle_amd import workloads as W
+    from testparticle_amd.device import DeviceEnsemble
+
+    out = bench_turbulent(args, torch, TP, W, DeviceEnsemble) if "headline" in sections else {}
+    sec = {}
+    if "dipole" in sections:
+        sec["dipole"] = bench_dipole(args, torch, TP, W, DeviceEnsemble)
+    if "batch" in sections:
+        sec["batch"] = bench_batch(args, torch, TP, W)
+    if "adjoint" in sections:
+        sec["adjoint"] = bench_adjoint(args, torch, TP, W, DeviceEnsemble)
+    if sec:
+        out["secondary"] = sec
     print(json.dumps(out), flush=True)
 
 

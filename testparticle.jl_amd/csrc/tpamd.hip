@@ -491,13 +491,14 @@ struct S32G : S32C {
 // Boris policies, so its trajectory is bit-identical to a solve of the same
 // problem; FP64 checkpoints are taken every K steps.
 // ---------------------------------------------------------------------------
-template <bool F32, bool PER>
+// FP32C: float node storage (the host checks it); FP64: node storage T.
+template <bool F32, bool PER, typename T = double>
 using AdjState = typename std::conditional<F32, S32G<kEUniform, 0, PER, false>,
-                                           S64G<kEUniform, 0, PER, double, false>>::type;
+                                           S64G<kEUniform, 0, PER, T, false>>::type;
 
-template <bool F32, bool PER>
+template <bool F32, bool PER, typename T>
 __device__ __forceinline__ void adj_ckpt_store(const AdjParams& a, long long N, int seg, long long i,
-                                               const AdjState<F32, PER>& s)
+                                               const AdjState<F32, PER, T>& s)
 {
     double* c = a.ckpt + (size_t)seg * 9 * N + i;
 #pragma unroll
@@ -513,9 +514,9 @@ __device__ __forceinline__ void adj_ckpt_store(const AdjParams& a, long long N, 
     }
 }
 
-template <bool F32, bool PER>
+template <bool F32, bool PER, typename T>
 __device__ __forceinline__ void adj_ckpt_load(const AdjParams& a, long long N, int seg, long long i,
-                                              AdjState<F32, PER>& s)
+                                              AdjState<F32, PER, T>& s)
 {
     const double* c = a.ckpt + (size_t)seg * 9 * N + i;
 #pragma unroll
@@ -528,14 +529,14 @@ __device__ __forceinline__ void adj_ckpt_load(const AdjParams& a, long long N, i
         } else {
             s.pos.x[d] = r;
         }
-        s.v[d] = (typename AdjState<F32, PER>::R)c[(size_t)(6 + d) * N];
+        s.v[d] = (typename AdjState<F32, PER, T>::R)c[(size_t)(6 + d) * N];
     }
 }
 
 // cell coordinates of the current position, as the forward gather computes them
-template <bool F32, bool PER>
-__device__ __forceinline__ void adj_cellcoord(const KParams& p, const AdjState<F32, PER>& s,
-                                              typename AdjState<F32, PER>::R sc[3])
+template <bool F32, bool PER, typename T>
+__device__ __forceinline__ void adj_cellcoord(const KParams& p, const AdjState<F32, PER, T>& s,
+                                              typename AdjState<F32, PER, T>::R sc[3])
 {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -546,20 +547,19 @@ __device__ __forceinline__ void adj_cellcoord(const KParams& p, const AdjState<F
 
 // one reverse step at the state held in s: (xbar, ubar) <- VJP of the step
 // that started there, with rotation factor h and position step dt
-template <bool F32, bool PER>
+template <bool F32, bool PER, typename T>
 __device__ __forceinline__ void adj_step_back(const KParams& p, const AdjParams& a,
-                                              const AdjState<F32, PER>& s, const typename AdjState<F32, PER>::R u[3],
-                                              double h, double dt, typename AdjState<F32, PER>::R xb[3],
-                                              typename AdjState<F32, PER>::R ub[3], bool with_dt)
+                                              const AdjState<F32, PER, T>& s, const typename AdjState<F32, PER, T>::R u[3],
+                                              double h, double dt, typename AdjState<F32, PER, T>::R xb[3],
+                                              typename AdjState<F32, PER, T>::R ub[3], bool with_dt)
 {
-    using R = typename AdjState<F32, PER>::R;
+    using R = typename AdjState<F32, PER, T>::R;
     R sc[3], w[3], c[8][3], B[3];
     int i0[3], i1[3];
     unsigned node[8];
-    adj_cellcoord<F32, PER>(p, s, sc);
+    adj_cellcoord<F32, PER, T>(p, s, sc);
     if (!adj_locate<PER, R>(p.B, sc, i0, i1, w)) return;
-    if (p.B.dtype == 1) adj_corners<float, R>(p.B, i0, i1, c, node);
-    else adj_corners<double, R>(p.B, i0, i1, c, node);
+    adj_corners<T, R>(p.B, i0, i1, c, node);
     adj_value<R>(c, w, B);
     const R hh = (R)h;
     R t[3], g[3], tb[3], un[3], Bb[3], xg[3];
@@ -579,10 +579,10 @@ __device__ __forceinline__ void adj_step_back(const KParams& p, const AdjParams&
     }
 }
 
-template <bool F32, bool PER>
+template <bool F32, bool PER, typename T>
 __global__ void __launch_bounds__(256) k_adj_forward(const KParams p, const AdjParams a)
 {
-    using S = AdjState<F32, PER>;
+    using S = AdjState<F32, PER, T>;
     using R = typename S::R;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double lsum = 0.0;
@@ -597,7 +597,7 @@ __global__ void __launch_bounds__(256) k_adj_forward(const KParams p, const AdjP
         for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
         double t = fma(-0.5, dt, p.t0);
         for (long long n = 0; n < p.nt; ++n) {
-            if (n % a.K == 0) adj_ckpt_store<F32, PER>(a, p.N, (int)(n / a.K), i, s);
+            if (n % a.K == 0) adj_ckpt_store<F32, PER, T>(a, p.N, (int)(n / a.K), i, s);
             t += dt;
             s.template U<0>(p, s.v, dt, t, vt);
 #pragma unroll
@@ -620,10 +620,10 @@ __global__ void __launch_bounds__(256) k_adj_forward(const KParams p, const AdjP
     if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(a.loss, lsum / (double)p.N);
 }
 
-template <bool F32, bool PER>
+template <bool F32, bool PER, typename T>
 __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjParams a)
 {
-    using S = AdjState<F32, PER>;
+    using S = AdjState<F32, PER, T>;
     using R = typename S::R;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.N) return;
@@ -642,7 +642,7 @@ __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjP
         const long long n0 = (long long)seg * a.K;
         const int len = (int)((p.nt - n0) < a.K ? (p.nt - n0) : a.K);
         S s;
-        adj_ckpt_load<F32, PER>(a, N, seg, i, s);
+        adj_ckpt_load<F32, PER, T>(a, N, seg, i, s);
         s.set_dt(dt);
         R vt[3];
         for (int j = 0; j < len; ++j) {                       // recompute the segment
@@ -679,14 +679,14 @@ __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjP
                     u[d] = b[(size_t)(3 + d) * N];
                 }
             }
-            adj_step_back<F32, PER>(p, a, s, u, h, dt, xb, ub, true);
+            adj_step_back<F32, PER, T>(p, a, s, u, h, dt, xb, ub, true);
         }
     }
     // the initial back half-step u(-1/2) = U(v0, x0, -dt/2) (boris.jl:514)
     S s0;
     s0.load(p, i);
     R v0[3] = {s0.v[0], s0.v[1], s0.v[2]};
-    adj_step_back<F32, PER>(p, a, s0, v0, (p.q2m * 0.5) * (-0.5 * dt), 0.0, xb, ub, false);
+    adj_step_back<F32, PER, T>(p, a, s0, v0, (p.q2m * 0.5) * (-0.5 * dt), 0.0, xb, ub, false);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         a.grad_xv0[6 * i + c] = (double)xb[c];
@@ -1714,13 +1714,20 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
     const int K = ckpt_interval;
     const int nseg = (int)((nt + K - 1) / K);
     const bool f32 = desc->precision == TPAMD_FP32C;
+    const bool bf32 = B->d.dtype == 1;
+    if (f32 && !bf32)
+        return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the FP32C adjoint needs float (f32) B storage");
+    // the forward pass runs the specialised pushers, which index the node array
+    // with 32-bit byte offsets
+    if (3.0 * (bf32 ? 4.0 : 8.0) * (double)B->d.n[0] * (double)B->d.n[1] * (double)B->d.n[2] >= 4294967296.0)
+        return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the adjoint needs a node array below 4 GiB");
     const int nc = f32 ? 9 : 6;
     const size_t rs = f32 ? 4 : 8;
     const size_t nnodes = (size_t)B->d.n[0] * B->d.n[1] * B->d.n[2];
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t b_xv = al(sizeof(double) * 6 * N), b_tg = al(sizeof(double) * 3 * N),
                  b_ck = al(sizeof(double) * 9 * (size_t)nseg * N), b_sg = al(rs * nc * (size_t)K * N),
-                 b_gb = al(sizeof(float) * 3 * nnodes), b_gx = al(sizeof(double) * 6 * N),
+                 b_gb = al(rs * 3 * nnodes), b_gx = al(sizeof(double) * 6 * N),
                  b_ls = al(sizeof(double)), b_xf = al(sizeof(double) * 3 * N);
     const size_t total = b_xv + b_tg + b_ck + b_sg + b_gb + b_gx + b_ls + b_xf;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -1732,7 +1739,7 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
     AdjParams a;
     a.ckpt = (double*)q; q += b_ck;
     a.seg = (void*)q; q += b_sg;
-    a.gradB = (float*)q; q += b_gb;
+    a.gradB = (void*)q; q += b_gb;
     a.grad_xv0 = (double*)q; q += b_gx;
     a.loss = (double*)q; q += b_ls;
     a.xfinal = (double*)q;
@@ -1748,17 +1755,21 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
     hipError_t e = hipMemcpyAsync(d_xv, xv0, sizeof(double) * 6 * N, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_tg, targets, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, a.gradB, ng);
+        if (f32) hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, (float*)a.gradB, ng);
+        else hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, (double*)a.gradB, ng);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, a.loss, 1LL);
         e = hipEventRecord(ctx->ev0, st);
     }
     if (e == hipSuccess) {
         if (f32) {
-            if (per) hipLaunchKernelGGL((k_adj_forward<true, true>), grid, block, 0, st, p, a);
-            else hipLaunchKernelGGL((k_adj_forward<true, false>), grid, block, 0, st, p, a);
+            if (per) hipLaunchKernelGGL((k_adj_forward<true, true, float>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_forward<true, false, float>), grid, block, 0, st, p, a);
+        } else if (bf32) {
+            if (per) hipLaunchKernelGGL((k_adj_forward<false, true, float>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_forward<false, false, float>), grid, block, 0, st, p, a);
         } else {
-            if (per) hipLaunchKernelGGL((k_adj_forward<false, true>), grid, block, 0, st, p, a);
-            else hipLaunchKernelGGL((k_adj_forward<false, false>), grid, block, 0, st, p, a);
+            if (per) hipLaunchKernelGGL((k_adj_forward<false, true, double>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_forward<false, false, double>), grid, block, 0, st, p, a);
         }
         e = hipGetLastError();
     }
@@ -1768,11 +1779,14 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, st);
     if (e == hipSuccess) {
         if (f32) {
-            if (per) hipLaunchKernelGGL((k_adj_reverse<true, true>), grid, block, 0, st, p, a);
-            else hipLaunchKernelGGL((k_adj_reverse<true, false>), grid, block, 0, st, p, a);
+            if (per) hipLaunchKernelGGL((k_adj_reverse<true, true, float>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_reverse<true, false, float>), grid, block, 0, st, p, a);
+        } else if (bf32) {
+            if (per) hipLaunchKernelGGL((k_adj_reverse<false, true, float>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_reverse<false, false, float>), grid, block, 0, st, p, a);
         } else {
-            if (per) hipLaunchKernelGGL((k_adj_reverse<false, true>), grid, block, 0, st, p, a);
-            else hipLaunchKernelGGL((k_adj_reverse<false, false>), grid, block, 0, st, p, a);
+            if (per) hipLaunchKernelGGL((k_adj_reverse<false, true, double>), grid, block, 0, st, p, a);
+            else hipLaunchKernelGGL((k_adj_reverse<false, false, double>), grid, block, 0, st, p, a);
         }
         e = hipGetLastError();
     }
@@ -1781,14 +1795,19 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
     if (e == hipSuccess) e = hipEventElapsedTime(&ms_r, ctx->ev0, ctx->ev1);
     std::vector<float> gb;
     if (e == hipSuccess) {
-        gb.resize(3 * nnodes);
-        e = hipMemcpy(gb.data(), a.gradB, sizeof(float) * 3 * nnodes, hipMemcpyDeviceToHost);
+        if (f32) {
+            gb.resize(3 * nnodes);
+            e = hipMemcpy(gb.data(), a.gradB, sizeof(float) * 3 * nnodes, hipMemcpyDeviceToHost);
+        } else {
+            e = hipMemcpy(dL_dB, a.gradB, sizeof(double) * 3 * nnodes, hipMemcpyDeviceToHost);
+        }
     }
     if (e == hipSuccess) e = hipMemcpy(dL_dxv0, a.grad_xv0, sizeof(double) * 6 * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(loss, a.loss, sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(buf);
     if (e != hipSuccess) return fail(ctx, TPAMD_ERR_DEVICE, "adjoint failed: %s", hipGetErrorString(e));
-    for (size_t j = 0; j < 3 * nnodes; ++j) dL_dB[j] = (double)gb[j];
+    if (f32)
+        for (size_t j = 0; j < 3 * nnodes; ++j) dL_dB[j] = (double)gb[j];
     if (stats) {
         stats->kernel_ms = ms_f;
         stats->aux_ms = ms_r;

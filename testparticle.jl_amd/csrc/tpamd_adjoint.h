@@ -29,7 +29,7 @@ struct AdjParams {
     const double* targets;    // [N][3]
     double* ckpt;             // [nseg][9][N]: frame position (3), wrap (3), velocity (3)
     void* seg;                // [K][9 or 6][N] recompute buffer (float or double)
-    float* gradB;             // node gradient, same layout as the node array [k][j][i][3]
+    void* gradB;              // node gradient [k][j][i][3]: float (FP32C) or double (FP64)
     double* grad_xv0;         // [N][6]
     double* loss;             // scalar sum
     double* xfinal;           // [N][3] final positions
@@ -141,10 +141,10 @@ __device__ __forceinline__ void adj_scatter(const AdjParams& a, const DevField& 
         const R ay = ((q >> 1) & 1) ? w[1] : (R)1 - w[1];
         const R az = (q >> 2) ? w[2] : (R)1 - w[2];
         const R W = ax * ay * az;
-        float* dst = a.gradB + 3u * node[q];
-        unsafeAtomicAdd(dst, (float)(W * Bb[0]));
-        unsafeAtomicAdd(dst + 1, (float)(W * Bb[1]));
-        unsafeAtomicAdd(dst + 2, (float)(W * Bb[2]));
+        R* dst = (R*)a.gradB + 3u * node[q];
+        unsafeAtomicAdd(dst, W * Bb[0]);
+        unsafeAtomicAdd(dst + 1, W * Bb[1]);
+        unsafeAtomicAdd(dst + 2, W * Bb[2]);
         const R cb = c[q][0] * Bb[0] + c[q][1] * Bb[1] + c[q][2] * Bb[2];
         const R sx = (q & 1) ? (R)1 : (R)-1;
         const R sy = ((q >> 1) & 1) ? (R)1 : (R)-1;
