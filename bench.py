@@ -431,13 +431,16 @@ def main():
     from testparticle_amd.device import DeviceEnsemble
 
     out = bench_turbulent(args, torch, TP, W, DeviceEnsemble) if "headline" in sections else {}
+    if out:
+        log("[bench] headline " + json.dumps(out))
     sec = {}
-    if "dipole" in sections:
-        sec["dipole"] = bench_dipole(args, torch, TP, W, DeviceEnsemble)
-    if "batch" in sections:
-        sec["batch"] = bench_batch(args, torch, TP, W)
-    if "adjoint" in sections:
-        sec["adjoint"] = bench_adjoint(args, torch, TP, W, DeviceEnsemble)
+    runners = {"dipole": lambda: bench_dipole(args, torch, TP, W, DeviceEnsemble),
+               "batch": lambda: bench_batch(args, torch, TP, W),
+               "adjoint": lambda: bench_adjoint(args, torch, TP, W, DeviceEnsemble)}
+    for name in SECTIONS[1:]:
+        if name in sections:
+            sec[name] = runners[name]()
+            log(f"[bench] {name} " + json.dumps(sec[name]))
     if sec:
         out["secondary"] = sec
     print(json.dumps(out), flush=True)

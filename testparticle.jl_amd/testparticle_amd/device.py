@@ -23,19 +23,19 @@ class DeviceEnsemble:
         self.E, self.B = _fields(prob)
         self.N = int(xv0.shape[0])
         self.desc = make_desc(prob, alg, dts=dts, **desc_kw)
+        self.dts = None
+        if dts is not None:
+            # per-particle steps: the schedule comes from nsteps, the pointer
+            # (device memory here) is only tested for presence by the planner
+            self.dts = torch.from_numpy(np.ascontiguousarray(dts, dtype=np.float64)).to(self.dev)
+            self.desc.dt_pp = self.dts.data_ptr()
         if alg.adaptive:
             cap = int(nout_cap if nout_cap is not None else 2)
         else:
-            if dts is not None:
-                self.desc.dt_pp = None
             cap = plan(self.desc)[1]
         self.desc.nout_cap = cap
         self.cap = cap
         self.xv0 = torch.from_numpy(np.ascontiguousarray(xv0, dtype=np.float64)).to(self.dev)
-        self.dts = None
-        if dts is not None:
-            self.dts = torch.from_numpy(np.ascontiguousarray(dts, dtype=np.float64)).to(self.dev)
-            self.desc.dt_pp = self.dts.data_ptr()
         self.u = torch.empty((self.N, max(cap, 1), 6), dtype=torch.float64, device=self.dev)
         self.t = torch.empty((self.N, max(cap, 1)), dtype=torch.float64, device=self.dev)
         self.len = torch.empty(self.N, dtype=torch.int32, device=self.dev)

@@ -262,6 +262,29 @@ def test_parity_dipole_belt():
     assert_bitwise(g, o)
 
 
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_device_ensemble_matches_host_solve(precision):
+    # device-resident buffers (tpamd_solve_device on torch's stream) against
+    # the host-buffer entry point, with per-particle dt and a removal sphere
+    pytest.importorskip("torch")
+    from testparticle_amd.device import DeviceEnsemble
+
+    w = W.dipole_belt(N=4096, nsteps=3000)
+    param = (w.q2m, TP.mi, TP.ZeroField(), TP.DipoleField(w.extra["b0"], w.extra["R"]), TP.ZeroField())
+    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), param)
+    kw = dict(dts=w.dts, nsteps=w.nsteps, save_everystep=False, isoutside=TP.InsideSphere(TP.Re),
+              precision=precision)
+    g = TP.solve(prob, TP.Boris(), **kw)
+    e = DeviceEnsemble(prob, TP.Boris(), w.xv0, **kw)
+    st = e.run()
+    u, t, ln, ret = e.host()
+    np.testing.assert_array_equal(ln, g.lengths)
+    np.testing.assert_array_equal(ret, g.retcodes)
+    np.testing.assert_array_equal(u[:, :g.u_all.shape[1]], g.u_all)
+    np.testing.assert_array_equal(t[:, :g.t_all.shape[1]], g.t_all)
+    assert st.particle_steps > 0
+
 @pytest.fixture(scope="module")
 def turb64():
     w, A = W.turbulent_box(N=1024, n=64, nsteps=2000, dtype=np.float64)
