@@ -274,13 +274,14 @@ def bench_batch(args, torch, TP, W):
     log(f"[bench] batch: generating 64 instances, ~{args.batch_total:.2e} protons")
     inst, tspan = W.batched_turbulence(n_inst=64, total=args.batch_total)
     probs = [TP.TraceProblem(xv, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(),
-                                         TP.GridField(A, *axes), TP.ZeroField()))
+                                         TP.GridField(A, *axes, layout=args.layout), TP.ZeroField()))
              for A, axes, xv in inst]
     N = sum(len(xv) for _, _, xv in inst)
     out = {"workload": "batched_turbulence_64", "instances": 64, "particles": N,
            "grid_edges": sorted({int(A.shape[1]) for A, _, _ in inst}),
            "grid_bytes_f32": int(sum(A.size * 4 for A, _, _ in inst)),
-           "alg": "AdaptiveBoris(safety=0.1)", "max_steps": 2000, "boundary": "exit at box faces"}
+           "alg": "AdaptiveBoris(safety=0.1)", "max_steps": 2000, "boundary": "exit at box faces",
+           "field_layout": args.layout}
     fin = {}
     for prec in ("fp32c", "fp64"):
         b = DeviceBatch(probs, TP.AdaptiveBoris(safety=0.1), precision=prec,
@@ -376,6 +377,21 @@ def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
     return out
 
 
+def gather_ceiling():
+    """Measured ceiling of the random cell-record gather (the program built
+    beside the library), run as a child process.  None if it is not built."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "testparticle.jl_amd", "_build", "gather_ceiling")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        log(f"[bench] gather_ceiling failed: {r.stderr.strip()}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def copy_probe(nbytes=1 << 30, reps=3):
     """A plain device copy of known size: under rocprofv3 --pmc it gives the
     counter reading (FETCH_SIZE / WRITE_SIZE) per byte actually moved, which
@@ -403,6 +419,8 @@ def main():
     ap.add_argument("--layout", choices=("cells", "nodes"), default="cells",
                     help="grid storage on the device (cells: one cache line per gather)")
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--no-ceiling", dest="ceiling", action="store_false",
+                    help="skip the measured gather ceiling")
     ap.add_argument("--no-accuracy", dest="accuracy", action="store_false",
                     help="skip the FP32C-vs-FP64 horizon runs")
     ap.add_argument("--sections", default=",".join(SECTIONS),
@@ -424,6 +442,11 @@ def main():
     if bad:
         ap.error(f"unknown sections {bad}")
 
+    ceiling = gather_ceiling() if args.ceiling and "headline" in sections else None
+    if ceiling:
+        log(f"[bench] measured gather ceiling: {ceiling['float_cell_gathers_per_s']:.3e} float-cell, "
+            f"{ceiling['double_cell_gathers_per_s']:.3e} double-cell gathers/s")
+
     import torch
 
     import testparticle_amd as TP
@@ -431,6 +454,12 @@ def main():
     from testparticle_amd.device import DeviceEnsemble
 
     out = bench_turbulent(args, torch, TP, W, DeviceEnsemble) if "headline" in sections else {}
+    if out and ceiling and args.layout == "cells":
+        # one cell-record gather per particle-step
+        out["roofline"]["measured_gather_ceiling"] = ceiling
+        out["roofline"]["frac_of_measured_ceiling"] = {
+            "fp32c": out["value"] / ceiling["float_cell_gathers_per_s"],
+            "fp64": out["fp64_value"] / ceiling["double_cell_gathers_per_s"]}
     if out:
         log("[bench] headline " + json.dumps(out))
     sec = {}

@@ -51,21 +51,28 @@ struct S64 {
         if (p.e_is_uxb) cross3(B, p.E.v, E);       // E = -u x B = B x u
         else field_f64(p.E, pos.x, t, E);
     }
-    // update_velocity (boris.jl:38-45) / MultistepUpdater (multistep_boris.jl:7-9)
+    // fields at the current position, then the velocity update from given
+    // fields: update_velocity (boris.jl:38-45) / MultistepUpdater
+    // (multistep_boris.jl:7-9) is U = eval + Uf
+    __device__ void eval(const KParams& p, double t, double E[3], double B[3]) const
+    {
+        fields(p, t, E, B);
+    }
+    template <int ALG>
+    __device__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
+                       double h, double out[3]) const
+    {
+        if (ALG == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
+        else multistep_f64(p, vin, E, B, h, out);
+    }
     template <int ALG>
     __device__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
     {
         double E[3], B[3];
-        fields(p, t, E, B);
-        if (ALG == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
-        else multistep_f64(p, vin, E, B, h, out);
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
     }
-    __device__ double bmag(const KParams& p, double t) const
-    {
-        double B[3];
-        field_f64(p.B, pos.x, t, B);
-        return sqrt(dot3(B, B));
-    }
+    __device__ double bmag_of(const double B[3]) const { return sqrt(dot3(B, B)); }
     __device__ void step(const double vv[3], double dt, Pos& xn) const
     {
 #pragma unroll
@@ -123,20 +130,25 @@ struct S32C {
             field_f32(p, p.E, q, t, E);
         }
     }
+    __device__ void eval(const KParams& p, double t, float E[3], float B[3]) const
+    {
+        fields(p, pos, t, E, B);
+    }
+    template <int ALG>
+    __device__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
+                       double h, float out[3]) const
+    {
+        if (ALG == 0) boris_f32(vin, E, B, (float)((p.q2m * 0.5) * h), out);
+        else multistep_f32(p, vin, E, B, (float)h, out);
+    }
     template <int ALG>
     __device__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
     {
         float E[3], B[3];
-        fields(p, pos, t, E, B);
-        if (ALG == 0) boris_f32(vin, E, B, (float)((p.q2m * 0.5) * h), out);
-        else multistep_f32(p, vin, E, B, (float)h, out);
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
     }
-    __device__ double bmag(const KParams& p, double t) const
-    {
-        float B[3];
-        field_f32(p, p.B, pos, t, B);
-        return (double)sqrtf(dot3f(B, B));
-    }
+    __device__ double bmag_of(const float B[3]) const { return (double)sqrtf(dot3f(B, B)); }
     __device__ void step_impl(const KParams& p, const float vv[3], Pos& xn) const
     {
 #pragma unroll
@@ -247,7 +259,9 @@ __device__ long long run_fixed(const KParams& p, long long i)
     for (; it <= nt; ++it) {
         R vprev[3] = {s.v[0], s.v[1], s.v[2]};
         t += dt;                                              // :454
-        s.template U<ALG>(p, s.v, dt, t, vt);                 // :455
+        R Ec[3], Bc[3];
+        s.eval(p, t, Ec, Bc);
+        s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vt);           // :455
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
         typename S::Pos xn;
@@ -263,7 +277,9 @@ __device__ long long run_fixed(const KParams& p, long long i)
             if (iout <= p.nout) {
                 const double tc = fma(-0.5, dt, t);
                 R vs[3];
-                s.template U<ALG>(p, vprev, 0.5 * dt, tc, vs);
+                // same position as the step's evaluation: static fields are re-used
+                if (p.static_fields) s.template Uf<ALG>(p, Ec, Bc, vprev, 0.5 * dt, vs);
+                else s.template U<ALG>(p, vprev, 0.5 * dt, tc, vs);
                 record(p, s, i, iout - 1, vs, tc);
                 tlast = tc;
             }
@@ -308,9 +324,15 @@ __device__ long long run_adaptive(const KParams& p, long long i)
         nrec += 1;
     };
     if (p.save_start) push(s.v, t);                           // :179-186
-    double dt = C / s.bmag(p, t);                             // :188-189
+    // Ec, Bc: the fields at the current (x, t).  The reference evaluates
+    // there up to four times per step (|B| for dt, both halves of the
+    // resynchronisation, a save); one evaluation serves them all, and with
+    // static fields the mid-step update as well.
+    R Ec[3], Bc[3];
+    s.eval(p, t, Ec, Bc);
+    double dt = C / s.bmag_of(Bc);                            // :188-189
     s.set_dt(dt);
-    s.template U<ALG>(p, s.v, -0.5 * dt, t, vt);              // :192
+    s.template Uf<ALG>(p, Ec, Bc, s.v, -0.5 * dt, vt);        // :192
 #pragma unroll
     for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
     long long it = 1;
@@ -327,19 +349,20 @@ __device__ long long run_adaptive(const KParams& p, long long i)
         }
         if (fabs(t + dt - t0) > fabs(ttotal)) {               // :198-203
             const double dts = t1 - t;
-            s.template U<ALG>(p, s.v, 0.5 * dt, t, vt);
-            s.template U<ALG>(p, vt, -0.5 * dts, t, s.v);
+            s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
+            s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dts, s.v);
             dt = dts;
             s.set_dt(dt);
         }
         if (it == next_save) {                                // :205-218
             next_save += p.K;
-            s.template U<ALG>(p, s.v, 0.5 * dt, t, vt);
+            s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
             push(vt, t);
         }
         const double tmid = t + 0.5 * dt;                     // :220
         R vn[3];
-        s.template U<ALG>(p, s.v, dt, tmid, vn);              // :221
+        if (p.static_fields) s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vn);   // :221
+        else s.template U<ALG>(p, s.v, dt, tmid, vn);
         typename S::Pos xn;
         do_step(p, s, vn, dt, xn);                            // :223
         const double tn = t + dt;
@@ -352,9 +375,10 @@ __device__ long long run_adaptive(const KParams& p, long long i)
         t = tn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.v[c] = vn[c];
-        const double dtn = C / s.bmag(p, t);                  // :237-238
-        s.template U<ALG>(p, s.v, 0.5 * dt, t, vt);           // :241-242
-        s.template U<ALG>(p, vt, -0.5 * dtn, t, s.v);
+        s.eval(p, t, Ec, Bc);
+        const double dtn = C / s.bmag_of(Bc);                 // :237-238
+        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);     // :241-242
+        s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dtn, s.v);
         dt = dtn;
         s.set_dt(dt);
         it += 1;
@@ -365,7 +389,7 @@ __device__ long long run_adaptive(const KParams& p, long long i)
         }
     }
     if (should && (nrec == 0 || tlast != t)) {                // :250-259
-        s.template U<ALG>(p, s.v, 0.5 * dt, t, vt);
+        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
         push(vt, t);
     }
     p.out_len[i] = (int)nrec;
@@ -427,21 +451,25 @@ struct S64G : S64 {
             trilinear<T, double>(p.B.data, ox, oy, oz, w, B);
         }
     }
+    __device__ void eval(const KParams& p, double, double E[3], double B[3]) const
+    {
+        bfield(p, B);
+        if (EMODE == kEUxB) cross3(B, p.E.v, E);
+        else { E[0] = p.E.v[0]; E[1] = p.E.v[1]; E[2] = p.E.v[2]; }
+    }
+    template <int ALG>
+    __device__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
+                       double h, double out[3]) const
+    {
+        if (NS == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
+        else multistep_f64_n<NS == 0 ? 1 : NS>(p, vin, E, B, h, out);
+    }
     template <int ALG>
     __device__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
     {
         double E[3], B[3];
-        bfield(p, B);
-        if (EMODE == kEUxB) cross3(B, p.E.v, E);
-        else { E[0] = p.E.v[0]; E[1] = p.E.v[1]; E[2] = p.E.v[2]; }
-        if (NS == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
-        else multistep_f64_n<NS == 0 ? 1 : NS>(p, vin, E, B, h, out);
-    }
-    __device__ double bmag(const KParams& p, double) const
-    {
-        double B[3];
-        bfield(p, B);
-        return sqrt(dot3(B, B));
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
     }
 };
 
@@ -468,21 +496,25 @@ struct S32G : S32C {
             trilinear<float, float>(p.B.data, ox, oy, oz, w, B);
         }
     }
+    __device__ void eval(const KParams& p, double, float E[3], float B[3]) const
+    {
+        bfield(p, B);
+        if (EMODE == kEUxB) cross3f(B, p.f32.u, E);
+        else { E[0] = p.f32.e0[0]; E[1] = p.f32.e0[1]; E[2] = p.f32.e0[2]; }
+    }
+    template <int ALG>
+    __device__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
+                       double h, float out[3]) const
+    {
+        if (NS == 0) boris_f32_fast(vin, E, B, (float)((p.q2m * 0.5) * h), out);
+        else multistep_f32_n<NS == 0 ? 1 : NS>(p, vin, E, B, (float)(p.q2m_2n * h), out);
+    }
     template <int ALG>
     __device__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
     {
         float E[3], B[3];
-        bfield(p, B);
-        if (EMODE == kEUxB) cross3f(B, p.f32.u, E);
-        else { E[0] = p.f32.e0[0]; E[1] = p.f32.e0[1]; E[2] = p.f32.e0[2]; }
-        if (NS == 0) boris_f32_fast(vin, E, B, (float)((p.q2m * 0.5) * h), out);
-        else multistep_f32_n<NS == 0 ? 1 : NS>(p, vin, E, B, (float)(p.q2m_2n * h), out);
-    }
-    __device__ double bmag(const KParams& p, double) const
-    {
-        float B[3];
-        bfield(p, B);
-        return (double)sqrtf(dot3f(B, B));
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
     }
 };
 
@@ -735,17 +767,23 @@ __global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table
 {
     const int lane = threadIdx.x & 63;
     long long steps = 0;
-    for (;;) {
-        unsigned c = 0;
-        if (lane == 0) c = atomicAdd(counter, 1u);
-        c = __shfl(c, 0, 64);
-        if (c >= (unsigned)nchunks) break;
+    // the chunk index is made wave-uniform (readfirstlane), so the chunk and
+    // the instance's parameter block are read with scalar loads; the next
+    // chunk is claimed before the current one is worked on, which hides the
+    // atomic's round trip
+    unsigned claim = 0;
+    if (lane == 0) claim = atomicAdd(counter, 1u);
+    unsigned c = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+    while (c < (unsigned)nchunks) {
+        unsigned next = 0;
+        if (lane == 0) next = atomicAdd(counter, 1u);
         const Chunk ch = chunks[c];
         const KParams& p = table[ch.inst];
         for (long long i = ch.begin + lane; i < ch.end; i += 64) {
             if constexpr (ADAPT) steps += run_adaptive<S, ALG>(p, i);
             else steps += run_fixed<S, ALG>(p, i);
         }
+        c = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
     }
     unsigned long long v = (unsigned long long)steps;
 #pragma unroll
@@ -923,6 +961,11 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
             return fail(ctx, TPAMD_ERR_UNSUPPORTED, "UxB electric field must derive from the solve's B");
         p.e_is_uxb = 1;
     }
+    // only the affine field can depend on t (its ft term); a UxB E follows B
+    auto is_static = [](const DevField& f) {
+        return f.kind != kFieldAffine || (f.ft[0] == 0.0 && f.ft[1] == 0.0 && f.ft[2] == 0.0);
+    };
+    p.static_fields = is_static(B->d) && (p.e_is_uxb || is_static(E->d)) ? 1 : 0;
     for (int d = 0; d < 3; ++d) { p.frame_origin[d] = 0.0; p.frame_period[d] = 0.0; }
     p.frame_periodic = 0;
     if (precision == TPAMD_FP32C) {
@@ -1286,7 +1329,11 @@ int batch_enqueue(tpamd_batch* b, hipStream_t st)
     select_policy(&b->desc, b->table[0], b->allow_spec, q);
     occ = std::max(occ, 1);
     const long long waves_needed = b->nchunks;
-    long long blocks = (long long)b->n_cu * occ;
+    // The occupancy query counts 256 VGPRs per lane, half of what gfx950
+    // gives a SIMD (e.g. 3 instead of 6 waves per SIMD at 80 VGPRs), so the
+    // persistent grid is doubled: blocks that find no free slot start when
+    // others leave, see an empty queue and exit.
+    long long blocks = (long long)b->n_cu * occ * 2;
     const long long waves_per_block = b->block_size / 64;
     blocks = std::min(blocks, (waves_needed + waves_per_block - 1) / waves_per_block);
     blocks = std::max(blocks, 1LL);

@@ -80,7 +80,9 @@ struct KParams {
     double q2m_2n;            // q/m / (2 n), multistep
     int alg, order_N, nsub, form;
     int save_start, save_end, save_every, bnd_flags;
-    int e_is_uxb, frame_periodic, pad0, pad1;
+    int e_is_uxb, frame_periodic;
+    int static_fields;        // neither E nor B depends on t
+    int pad1;
     double q2m, t0, t1, dt, C;
     long long nt, nout, cap, K, max_steps, N;
     double r_in, r_out;

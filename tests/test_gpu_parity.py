@@ -416,6 +416,26 @@ def test_parity_affine_time_dependent_all_orders():
         assert_bitwise(g, o)
 
 
+
+@pytest.mark.parametrize("ft", [0.0, 0.05])
+def test_parity_affine_adaptive_static_and_time_dependent(ft):
+    # ft = 0: static fields, one evaluation per adaptive step serves every
+    # update; ft != 0: the mid-step update evaluates at t + dt/2 on its own
+    rng = np.random.default_rng(6)
+    N = 256
+    xv0 = np.hstack([rng.normal(size=(N, 3)), rng.normal(size=(N, 3))])
+    E = TP.AffineField([0.1, -0.2, 0.05], rng.normal(size=9) * 0.05, [0.5 * ft, 0.0, -ft])
+    B = TP.AffineField([0.0, 0.3, 1.0], rng.normal(size=9) * 0.1, [0.0, ft, 0.0])
+    prob = TP.TraceProblem(xv0, (0.5, 20.5), (1.3, 1.0, E, B, TP.ZeroField()))
+    kw = dict(savestepinterval=5, max_steps=3000, nout_cap=800)
+    for alg in (TP.AdaptiveBoris(safety=0.05), TP.AdaptiveMultistepBoris[4](n=2)):
+        g = TP.solve(prob, alg, **kw)
+        o = run_oracle(prob, alg, **kw)
+        assert_bitwise(g, o)
+        g = TP.solve(prob, alg, save_everystep=False, **kw)
+        o = run_oracle(prob, alg, save_everystep=False, **kw)
+        assert_bitwise(g, o)
+
 # ---------------------------------------------------------------------------
 # FP32C: error against the FP64 path
 # ---------------------------------------------------------------------------
