@@ -345,19 +345,34 @@ def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
         f_ms = float(np.mean([r.forward_ms for r in rs]))
         r_ms = float(np.mean([r.reverse_ms for r in rs]))
         best = min(fwd.values())
+        g = rs[-1]
         out[prec] = {"forward_solve_ms": fwd, "adjoint_forward_ms": f_ms, "adjoint_reverse_ms": r_ms,
-                     "cost_over_forward": (f_ms + r_ms) / best, "loss": rs[-1].loss}
-        out[prec]["_grad"] = rs[-1]
+                     "cost_over_forward": (f_ms + r_ms) / best, "loss": g.loss,
+                     "max_abs_dL_dB": float(np.nanmax(np.abs(g.dL_dB))),
+                     "max_abs_dL_dxv0": float(np.nanmax(np.abs(g.dL_dxv0))),
+                     "nonfinite_gradients": int(np.sum(~np.isfinite(g.dL_dB)) + np.sum(~np.isfinite(g.dL_dxv0)))}
         log(f"[bench] adjoint {prec}: forward solve {best:.2f} ms, adjoint {f_ms:.2f} + {r_ms:.2f} ms "
             f"= {(f_ms + r_ms) / best:.2f}x")
 
-    # dL/dB on 32 nodes against central differences of the FP64 device solve
+    # Gradient check against central differences of the FP64 device solve on
+    # 32 nodes.  Over 2,048 steps this flow amplifies perturbations by many
+    # orders of magnitude (the exact gradient grows like exp(lambda T)), so the
+    # check runs over one checkpoint segment, where the linearisation holds.
+    hsteps = K
+    tspan_h = (0.0, hsteps * dt)
+
+    def prob_h(Av, storage):
+        B = TP.GridField(Av, *ax, bc=TP.PeriodicBC(), storage=storage)
+        return TP.TraceProblem(xv0, tspan_h, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+
     def loss64(Av):
-        xT = TP.solve(prob_of(Av, "f64"), TP.Boris(), dt=dt, save_everystep=False,
+        xT = TP.solve(prob_h(Av, "f64"), TP.Boris(), dt=dt, save_everystep=False,
                       save_start=False).final_states()[:, :3]
         return float(np.mean(np.sum((xT - tg) ** 2, axis=1)))
 
-    g64 = out["fp64"]["_grad"].dL_dB
+    gh = {prec: TP.loss_gradient(prob_h(A, storage), dt=dt, targets=tg, precision=prec, ckpt_interval=16)
+          for prec, storage in (("fp32c", "f32"), ("fp64", "f64"))}
+    g64 = gh["fp64"].dL_dB
     cand = np.argwhere(np.abs(g64) > 1e-3 * np.abs(g64).max())
     picks = cand[np.random.default_rng(57).choice(len(cand), 32, replace=False)]
     h = 1e-6 * B0
@@ -369,10 +384,10 @@ def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
         fd.append((loss64(Ap) - loss64(Am)) / (2 * h))
     fd = np.array(fd)
     for prec in ("fp32c", "fp64"):
-        g = out[prec].pop("_grad").dL_dB
-        ga = np.array([g[c, i, j, k] for c, i, j, k in picks])
+        ga = np.array([gh[prec].dL_dB[c, i, j, k] for c, i, j, k in picks])
         rel = np.abs(ga - fd) / np.abs(fd)
-        out[prec]["dL_dB_vs_central_fd"] = {"nodes": 32, "h": h, "median_rel": float(np.median(rel)),
+        out[prec]["dL_dB_vs_central_fd"] = {"steps": hsteps, "nodes": 32, "h": h,
+                                            "median_rel": float(np.median(rel)),
                                             "max_rel": float(np.max(rel))}
     return out
 

@@ -19,22 +19,48 @@
 // Memory: FP64 checkpoints of the state every K steps; the reverse sweep
 // recomputes each K-step segment into a per-particle buffer (structure of
 // arrays, particle index fastest) and walks it backwards.
+//
+// dL/dB without global atomics: each reverse step writes one record (the
+// cell it sampled, Bbar and the three trilinear weights).  Per segment the
+// records are radix-partitioned by cell bin, and one workgroup per bin sums
+// its records into an LDS tile of per-cell corner moments (8 corners x 3
+// components), which it alone adds into the global per-cell gradient.  A
+// last pass folds the per-cell corner moments into the node gradient.
 #pragma once
 
 #include "tpamd_grid.h"
 
 namespace tpamd {
 
+// one reverse-step contribution: Bbar and the cell weights (wx, wy, wz)
+template <typename R>
+struct AdjRec {
+    R bb[3];
+    R w[3];
+};
+
+// cells per bin: the bin's LDS tile holds 24 values per cell, 96 KiB
+template <typename R>
+struct AdjBin {
+    static constexpr int kShift = sizeof(R) == 4 ? 10 : 9;
+    static constexpr int kCells = 1 << kShift;
+    static constexpr int kThreads = 1024;
+};
+
 struct AdjParams {
     const double* targets;    // [N][3]
     double* ckpt;             // [nseg][9][N]: frame position (3), wrap (3), velocity (3)
     void* seg;                // [K][9 or 6][N] recompute buffer (float or double)
-    void* gradB;              // node gradient [k][j][i][3]: float (FP32C) or double (FP64)
+    void* state;              // [6][N] adjoint state (xbar, ubar) between segment launches
+    unsigned* rec_key;        // [(K + 1) N] record keys: the cell index
+    void* rec_val;            // [(K + 1) N] AdjRec<R>; rows without a step hold a zero record
+    unsigned ncell;           // cells in the grid
     double* grad_xv0;         // [N][6]
     double* loss;             // scalar sum
     double* xfinal;           // [N][3] final positions
     int K;                    // checkpoint interval
     int nseg;
+    int nc[3];                // cells per axis (n periodic, n - 1 otherwise)
 };
 
 // ---------------------------------------------------------------------------
@@ -109,6 +135,20 @@ __device__ __forceinline__ void adj_corners(const DevField& f, const int i0[3], 
     }
 }
 
+// the 8 corner vectors from the cell-replicated copy (corner q = (qx, qy, qz)
+// in bits 0..2, 3 components each, as the cell builder stores them)
+template <typename T, typename R>
+__device__ __forceinline__ void adj_corners_cell(const DevField& f, const int i0[3], R c[8][3])
+{
+    const T* rec = (const T*)((const char*)f.cells + cell_offset(f, i0));
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        c[q][0] = (R)rec[3 * q];
+        c[q][1] = (R)rec[3 * q + 1];
+        c[q][2] = (R)rec[3 * q + 2];
+    }
+}
+
 // trilinear value, blended in the forward kernels' order (lerp along x, y, z)
 template <typename R>
 __device__ __forceinline__ void adj_value(const R c[8][3], const R w[3], R B[3])
@@ -128,11 +168,10 @@ __device__ __forceinline__ void adj_value(const R c[8][3], const R w[3], R B[3])
     }
 }
 
-// scatter Bbar to the corner nodes and return (dB/dx)^T Bbar
+// (dB/dx)^T Bbar, the trilinear gradient through the cell weights
 template <typename R>
-__device__ __forceinline__ void adj_scatter(const AdjParams& a, const DevField& f, const R c[8][3],
-                                            const unsigned node[8], const R w[3], const R Bb[3],
-                                            R xb[3])
+__device__ __forceinline__ void adj_xgrad(const DevField& f, const R c[8][3], const R w[3],
+                                          const R Bb[3], R xb[3])
 {
     R g[3] = {0, 0, 0};
 #pragma unroll
@@ -140,11 +179,6 @@ __device__ __forceinline__ void adj_scatter(const AdjParams& a, const DevField& 
         const R ax = (q & 1) ? w[0] : (R)1 - w[0];
         const R ay = ((q >> 1) & 1) ? w[1] : (R)1 - w[1];
         const R az = (q >> 2) ? w[2] : (R)1 - w[2];
-        const R W = ax * ay * az;
-        R* dst = (R*)a.gradB + 3u * node[q];
-        unsafeAtomicAdd(dst, W * Bb[0]);
-        unsafeAtomicAdd(dst + 1, W * Bb[1]);
-        unsafeAtomicAdd(dst + 2, W * Bb[2]);
         const R cb = c[q][0] * Bb[0] + c[q][1] * Bb[1] + c[q][2] * Bb[2];
         const R sx = (q & 1) ? (R)1 : (R)-1;
         const R sy = ((q >> 1) & 1) ? (R)1 : (R)-1;
@@ -155,6 +189,88 @@ __device__ __forceinline__ void adj_scatter(const AdjParams& a, const DevField& 
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) xb[d] = g[d] * (R)f.inv_h[d];
+}
+
+// Sum the records of one cell bin (keys sorted on the bin bits; the bin's
+// range is found by binary search) into an LDS tile of per-cell corner
+// moments, then add the tile into the global per-cell gradient G[cell][24],
+// which this workgroup alone writes.  Corner q = (qx, qy, qz) in bits 0..2.
+// The tile is component-major, tile[(3 q + c) * kCells + cell], so the lanes
+// of one LDS atomic spread over the banks.
+template <typename R>
+__global__ void __launch_bounds__(AdjBin<R>::kThreads) k_adj_bins(const unsigned* __restrict__ keys,
+                                                                  const AdjRec<R>* __restrict__ vals,
+                                                                  long long M, unsigned ncell,
+                                                                  R* __restrict__ G)
+{
+    constexpr int kShift = AdjBin<R>::kShift, kCells = AdjBin<R>::kCells;
+    __shared__ R tile[kCells * 24];
+    const unsigned bin = blockIdx.x;
+    const unsigned first = bin << kShift;
+    auto lower = [&](unsigned target) {
+        long long lo = 0, hi = M;
+        while (lo < hi) {
+            const long long mid = (lo + hi) >> 1;
+            if (keys[mid] < target) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const long long r0 = lower(first), r1 = lower(first + (unsigned)kCells);
+    for (int k = threadIdx.x; k < kCells * 24; k += blockDim.x) tile[k] = (R)0;
+    __syncthreads();
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const unsigned local = keys[r] - first;
+        const AdjRec<R> v = vals[r];
+        R* t = tile + local;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const R W = ((q & 1) ? v.w[0] : (R)1 - v.w[0]) *
+                        (((q >> 1) & 1) ? v.w[1] : (R)1 - v.w[1]) *
+                        ((q >> 2) ? v.w[2] : (R)1 - v.w[2]);
+            atomicAdd(t + (3 * q) * kCells, W * v.bb[0]);
+            atomicAdd(t + (3 * q + 1) * kCells, W * v.bb[1]);
+            atomicAdd(t + (3 * q + 2) * kCells, W * v.bb[2]);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kCells * 24; k += blockDim.x) {
+        const unsigned local = (unsigned)(k / 24), comp = (unsigned)(k % 24);
+        const unsigned cell = first + local;
+        if (cell < ncell) G[(size_t)cell * 24 + comp] += tile[comp * kCells + local];
+    }
+}
+
+// node gradient [k][j][i][3] from the per-cell corner moments: node (i, j, k)
+// is corner q of cell (i - qx, j - qy, k - qz), wrapped on periodic grids
+template <typename R>
+__global__ void __launch_bounds__(256) k_adj_nodes(const R* __restrict__ G, int nx, int ny, int nz,
+                                                   int cx, int cy, int cz, int periodic,
+                                                   R* __restrict__ out)
+{
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nn = (long long)nx * ny * nz;
+    if (id >= nn) return;
+    const int i = (int)(id % nx), j = (int)((id / nx) % ny), k = (int)(id / ((long long)nx * ny));
+    R acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        int ci = i - (q & 1), cj = j - ((q >> 1) & 1), ck = k - (q >> 2);
+        if (periodic) {
+            ci += ci < 0 ? nx : 0;
+            cj += cj < 0 ? ny : 0;
+            ck += ck < 0 ? nz : 0;
+        } else if (ci < 0 || cj < 0 || ck < 0 || ci >= cx || cj >= cy || ck >= cz) {
+            continue;
+        }
+        const R* g = G + ((size_t)ci + (size_t)cx * ((size_t)cj + (size_t)cy * (size_t)ck)) * 24 + 3 * q;
+        acc[0] += g[0];
+        acc[1] += g[1];
+        acc[2] += g[2];
+    }
+    out[3 * id] = acc[0];
+    out[3 * id + 1] = acc[1];
+    out[3 * id + 2] = acc[2];
 }
 
 }  // namespace tpamd

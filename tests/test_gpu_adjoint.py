@@ -102,3 +102,60 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     for r in rs[1:]:
         np.testing.assert_array_equal(r.dL_dxv0, rs[0].dL_dxv0)
         np.testing.assert_allclose(r.dL_dB, rs[0].dL_dB, rtol=1e-5, atol=1e-7 * np.abs(rs[0].dL_dB).max())
+
+
+@pytest.mark.parametrize("precision,storage", [("fp64", "f64"), ("fp64", "f32"), ("fp32c", "f32")])
+def test_grid_gradient_clamped_grid(precision, storage):
+    # a non-periodic grid (n - 1 cells per axis, clamped outside) and FP64
+    # arithmetic over float-stored nodes
+    n, N, nsteps = 10, 1024, 40
+    A = W.turbulent_field(n, L_BOX, 5e-9, 0.3, 9, dtype=np.float64)
+    if storage == "f32":
+        A = A.astype(np.float32).astype(np.float64)     # the values float storage holds exactly
+    ax = W.grid_axes(n, L_BOX, periodic=False)
+    rng = np.random.default_rng(10)
+    x = rng.uniform(0.2 * L_BOX, 0.8 * L_BOX, size=(N, 3))
+    v = W.isotropic_directions(rng, N) * np.sqrt(2 * 1.0e3 * TP.qi / TP.mi)
+    xv0 = np.hstack([x, v])
+    dt = 2 * np.pi * TP.mi / (TP.qi * 5e-9) / 40
+    tspan = (0.0, nsteps * dt)
+    tg = rng.uniform(0, L_BOX, size=(N, 3))
+
+    def prob_of(Av, st):
+        B = TP.GridField(Av, *ax, bc=TP.ClampExtrap(), storage=st)
+        return TP.TraceProblem(xv0, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+
+    def loss64(Av):
+        xT = TP.solve(prob_of(Av, "f64"), TP.Boris(), dt=dt, save_everystep=False).final_states()[:, :3]
+        return float(np.mean(np.sum((xT - tg) ** 2, axis=1)))
+
+    r = TP.loss_gradient(prob_of(A, storage), dt=dt, targets=tg, precision=precision, ckpt_interval=16)
+    assert r.loss == pytest.approx(loss64(A), rel=1e-5 if precision == "fp32c" else 1e-13)
+    g = np.abs(r.dL_dB)
+    cand = np.argwhere(g > 1e-3 * g.max())
+    picks = cand[np.random.default_rng(12).choice(len(cand), 16, replace=False)]
+    h = 1e-6 * 5e-9
+    worst = 0.0
+    for c, i, j, k in picks:
+        Ap, Am = A.copy(), A.copy()
+        Ap[c, i, j, k] += h
+        Am[c, i, j, k] -= h
+        fd = (loss64(Ap) - loss64(Am)) / (2 * h)
+        worst = max(worst, abs(r.dL_dB[c, i, j, k] - fd) / abs(fd))
+    print(f"{precision}/{storage} clamped: worst relative error of dL/dB vs central FD {worst:.2e}")
+    assert worst < (1e-4 if precision == "fp32c" else 1e-5)
+
+
+@pytest.mark.parametrize("precision,storage", [("fp64", "f64"), ("fp32c", "f32")])
+def test_cell_layout_adjoint_matches_nodes(precision, storage):
+    A, ax, xv0, dt, tspan, tg = _setup(N=1000, nsteps=70)
+    rs = []
+    for layout in ("nodes", "cells"):
+        B = TP.GridField(A, *ax, bc=TP.PeriodicBC(), storage=storage, layout=layout)
+        prob = TP.TraceProblem(xv0, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+        rs.append(TP.loss_gradient(prob, dt=dt, targets=tg, precision=precision, ckpt_interval=16))
+    # the loss is a sum of per-wave partial sums added atomically: last-bit order effects
+    assert rs[1].loss == pytest.approx(rs[0].loss, rel=1e-13)
+    np.testing.assert_array_equal(rs[0].dL_dxv0, rs[1].dL_dxv0)
+    scale = np.abs(rs[0].dL_dB).max()
+    np.testing.assert_allclose(rs[1].dL_dB, rs[0].dL_dB, rtol=1e-5, atol=1e-6 * scale)
