@@ -323,7 +323,7 @@ def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
     tspan = (0.0, nsteps * dt)
     log(f"[bench] adjoint: {n}^3 grid, {N} protons x {nsteps} steps, checkpoints every {K}")
 
-    def prob_of(Av, storage, layout="nodes"):
+    def prob_of(Av, storage, layout="cells"):
         B = TP.GridField(Av, *ax, bc=TP.PeriodicBC(), storage=storage, layout=layout)
         return TP.TraceProblem(xv0, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
 

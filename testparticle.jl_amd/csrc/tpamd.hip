@@ -1478,11 +1478,11 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
         return fail(ctx, TPAMD_ERR_CAPACITY, "too many adjoint records: lower the checkpoint interval or N");
     // records are partitioned on the bin bits [kShift, cellbits) of the cell
     // index; a grid of one bin needs no partition
-    constexpr int kShift = AdjBin<R>::kShift;
+    constexpr int kShift = AdjBin::kShift;
     int cellbits = 0;
     while ((1u << cellbits) < ncell) ++cellbits;
     const bool partition = cellbits > kShift;
-    const unsigned nbins = (ncell + AdjBin<R>::kCells - 1) / AdjBin<R>::kCells;
+    const unsigned nbins = (ncell + AdjBin::kCells - 1) / AdjBin::kCells;
     size_t temp_bytes = 0;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (partition)
@@ -1493,7 +1493,7 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
                  b_ck = al(sizeof(double) * 9 * (size_t)nseg * N), b_sg = al(rs * ncomp * (size_t)K * N),
                  b_st = al(rs * 6 * (size_t)N), b_key = al(sizeof(unsigned) * (size_t)M),
                  b_val = al(sizeof(AdjRec<R>) * (size_t)M), b_tmp = al(std::max<size_t>(temp_bytes, 1)),
-                 b_G = al(rs * 24 * (size_t)ncell), b_out = al(rs * 3 * nnodes),
+                 b_G = al(sizeof(double) * 24 * (size_t)ncell), b_out = al(sizeof(double) * 3 * nnodes),
                  b_gx = al(sizeof(double) * 6 * N), b_ls = al(sizeof(double)), b_xf = al(sizeof(double) * 3 * N);
     const size_t total = b_xv + b_tg + b_ck + b_sg + b_st + 2 * b_key + 2 * b_val + b_tmp + b_G + b_out +
                          b_gx + b_ls + b_xf;
@@ -1512,8 +1512,8 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     AdjRec<R>* vals0 = (AdjRec<R>*)take(b_val);
     AdjRec<R>* vals1 = (AdjRec<R>*)take(b_val);
     void* d_tmp = take(b_tmp);
-    R* G = (R*)take(b_G);
-    R* d_out = (R*)take(b_out);
+    double* G = (double*)take(b_G);
+    double* d_out = (double*)take(b_out);
     a.grad_xv0 = (double*)take(b_gx);
     a.loss = (double*)take(b_ls);
     a.xfinal = (double*)take(b_xf);
@@ -1532,10 +1532,7 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     hipError_t e = hipMemcpyAsync(d_xv, xv0, sizeof(double) * 6 * N, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(d_tg, targets, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        if constexpr (sizeof(R) == 4)
-            hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, (float*)G, ng);
-        else
-            hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, (double*)G, ng);
+        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, G, ng);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, a.loss, 1LL);
         e = hipEventRecord(ctx->ev0, st);
     }
@@ -1553,29 +1550,24 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
         if (e == hipSuccess && partition)
             e = adj_sort_records<R>(d_tmp, temp_bytes, keys0, keys1, vals0, vals1, M, kShift, cellbits, st);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL((k_adj_bins<R>), dim3(nbins), dim3(AdjBin<R>::kThreads), 0, st,
+            hipLaunchKernelGGL((k_adj_bins<R>), dim3(nbins), dim3(AdjBin::kThreads), 0, st,
                                partition ? keys1 : keys0, partition ? vals1 : vals0, M, ncell, G);
             e = hipGetLastError();
         }
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL((k_adj_nodes<R>), dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, st, G, nx,
+        hipLaunchKernelGGL(k_adj_nodes<double>, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, st, G, nx,
                            ny, nz, nc[0], nc[1], nc[2], PER ? 1 : 0, d_out);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms_r, ctx->ev0, ctx->ev1);
-    std::vector<R> gb;
-    if (e == hipSuccess) {
-        gb.resize(3 * nnodes);
-        e = hipMemcpy(gb.data(), d_out, rs * 3 * nnodes, hipMemcpyDeviceToHost);
-    }
+    if (e == hipSuccess) e = hipMemcpy(dL_dB, d_out, sizeof(double) * 3 * nnodes, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(dL_dxv0, a.grad_xv0, sizeof(double) * 6 * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(loss, a.loss, sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(buf);
     if (e != hipSuccess) return fail(ctx, TPAMD_ERR_DEVICE, "adjoint failed: %s", hipGetErrorString(e));
-    for (size_t j = 0; j < 3 * nnodes; ++j) dL_dB[j] = (double)gb[j];
     if (stats) {
         stats->kernel_ms = ms_f;
         stats->aux_ms = ms_r;

@@ -39,10 +39,11 @@ struct AdjRec {
     R w[3];
 };
 
-// cells per bin: the bin's LDS tile holds 24 values per cell, 96 KiB
-template <typename R>
+// cells per bin: the bin's LDS tile holds 24 doubles per cell, 96 KiB.  The
+// tile is double for both precisions: on gfx950 an LDS float atomic add
+// (ds_add_f32) runs at ~1/7 the rate of the double one (ds_add_f64).
 struct AdjBin {
-    static constexpr int kShift = sizeof(R) == 4 ? 10 : 9;
+    static constexpr int kShift = 9;
     static constexpr int kCells = 1 << kShift;
     static constexpr int kThreads = 1024;
 };
@@ -198,13 +199,13 @@ __device__ __forceinline__ void adj_xgrad(const DevField& f, const R c[8][3], co
 // The tile is component-major, tile[(3 q + c) * kCells + cell], so the lanes
 // of one LDS atomic spread over the banks.
 template <typename R>
-__global__ void __launch_bounds__(AdjBin<R>::kThreads) k_adj_bins(const unsigned* __restrict__ keys,
-                                                                  const AdjRec<R>* __restrict__ vals,
-                                                                  long long M, unsigned ncell,
-                                                                  R* __restrict__ G)
+__global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins(const unsigned* __restrict__ keys,
+                                                               const AdjRec<R>* __restrict__ vals,
+                                                               long long M, unsigned ncell,
+                                                               double* __restrict__ G)
 {
-    constexpr int kShift = AdjBin<R>::kShift, kCells = AdjBin<R>::kCells;
-    __shared__ R tile[kCells * 24];
+    constexpr int kShift = AdjBin::kShift, kCells = AdjBin::kCells;
+    __shared__ double tile[kCells * 24];
     const unsigned bin = blockIdx.x;
     const unsigned first = bin << kShift;
     auto lower = [&](unsigned target) {
@@ -217,20 +218,20 @@ __global__ void __launch_bounds__(AdjBin<R>::kThreads) k_adj_bins(const unsigned
         return lo;
     };
     const long long r0 = lower(first), r1 = lower(first + (unsigned)kCells);
-    for (int k = threadIdx.x; k < kCells * 24; k += blockDim.x) tile[k] = (R)0;
+    for (int k = threadIdx.x; k < kCells * 24; k += blockDim.x) tile[k] = 0.0;
     __syncthreads();
     for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const unsigned local = keys[r] - first;
         const AdjRec<R> v = vals[r];
-        R* t = tile + local;
+        double* t = tile + local;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const R W = ((q & 1) ? v.w[0] : (R)1 - v.w[0]) *
                         (((q >> 1) & 1) ? v.w[1] : (R)1 - v.w[1]) *
                         ((q >> 2) ? v.w[2] : (R)1 - v.w[2]);
-            atomicAdd(t + (3 * q) * kCells, W * v.bb[0]);
-            atomicAdd(t + (3 * q + 1) * kCells, W * v.bb[1]);
-            atomicAdd(t + (3 * q + 2) * kCells, W * v.bb[2]);
+            atomicAdd(t + (3 * q) * kCells, (double)(W * v.bb[0]));
+            atomicAdd(t + (3 * q + 1) * kCells, (double)(W * v.bb[1]));
+            atomicAdd(t + (3 * q + 2) * kCells, (double)(W * v.bb[2]));
         }
     }
     __syncthreads();
