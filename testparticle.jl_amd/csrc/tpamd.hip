@@ -203,9 +203,17 @@ __device__ __forceinline__ bool outside(const KParams& p, const S& s, const type
             if (x[c] != x[c] || v[c] != v[c]) return true;
     }
     if (f & (kBndSphereIn | kBndSphereOut)) {
-        const double r = sqrt(dot3(x, x));
-        if ((f & kBndSphereIn) && r < p.r_in) return true;
-        if ((f & kBndSphereOut) && r > p.r_out) return true;
+        // r = sqrt(|x|^2) compared with the radius; away from the sphere the
+        // squared norm alone decides it, near it the exact test runs
+        const double d = dot3(x, x);
+        if (f & kBndSphereIn) {
+            if (d < p.rin2_lo) return true;
+            if (d <= p.rin2_hi && sqrt(d) < p.r_in) return true;
+        }
+        if (f & kBndSphereOut) {
+            if (d > p.rout2_hi) return true;
+            if (d >= p.rout2_lo && sqrt(d) > p.r_out) return true;
+        }
     }
     if (f & kBndBox) {
 #pragma unroll
@@ -1267,6 +1275,12 @@ int fill_params(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field*
     p.N = N;
     p.r_in = desc->r_in;
     p.r_out = desc->r_out;
+    // sqrt is correctly rounded and monotone: |x|^2 < (r (1 - 1e-12))^2 gives
+    // sqrt(|x|^2) < r, and |x|^2 > (r (1 + 1e-12))^2 gives sqrt(|x|^2) > r
+    p.rin2_lo = (p.r_in * (1.0 - 1e-12)) * (p.r_in * (1.0 - 1e-12));
+    p.rin2_hi = (p.r_in * (1.0 + 1e-12)) * (p.r_in * (1.0 + 1e-12));
+    p.rout2_lo = (p.r_out * (1.0 - 1e-12)) * (p.r_out * (1.0 - 1e-12));
+    p.rout2_hi = (p.r_out * (1.0 + 1e-12)) * (p.r_out * (1.0 + 1e-12));
     for (int d = 0; d < 3; ++d) { p.box_lo[d] = desc->box_lo[d]; p.box_hi[d] = desc->box_hi[d]; }
     if (desc->bnd_flags & TPAMD_BND_GRID) {
         // open boundary at the faces of B's grid: the node span becomes the box

@@ -86,6 +86,9 @@ struct KParams {
     double q2m, t0, t1, dt, C;
     long long nt, nout, cap, K, max_steps, N;
     double r_in, r_out;
+    // |x|^2 bounds around each sphere: outside [lo, hi] the radius test is
+    // decided without the square root (same outcome as sqrt(|x|^2) vs r)
+    double rin2_lo, rin2_hi, rout2_lo, rout2_hi;
     double box_lo[3], box_hi[3];
     double frame_origin[3];   // FP32C: positions stored relative to this point
     double frame_period[3];   // FP32C: wrap period (0 = no wrap)
