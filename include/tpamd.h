@@ -125,8 +125,16 @@ typedef struct tpamd_solve_desc {
     int64_t nsteps_pp;
     int32_t block_size;       /* 0 = library default */
     int32_t flags;            /* TPAMD_FLAG_* bits */
-    int32_t reserved[6];
+    double mass;              /* p[2] = m, for save_work (src/boris/boris.jl:405-440) */
+    int32_t save_fields;      /* records gain E(x, t) and B(x, t): 6 + 6 columns */
+    int32_t save_work;        /* records gain (P_par, P_fermi, P_grad, P_betatron) */
+    int32_t reserved[2];
 } tpamd_solve_desc;
+
+/* Columns of one record: 6 (x, v) + 6 when save_fields + 4 when save_work,
+ * in that order, as _prepare_saved_data builds them (src/boris/boris.jl:405-440).
+ * Record buffers are [N][nout_cap][record width]. */
+#define TPAMD_RECORD_WIDTH(d) (6 + ((d)->save_fields ? 6 : 0) + ((d)->save_work ? 4 : 0))
 
 /* descriptor flags */
 #define TPAMD_FLAG_GENERIC_KERNELS 1  /* skip the gridded-field specialisations (for cross-checks) */
@@ -183,6 +191,14 @@ int32_t tpamd_field_eval(tpamd_ctx* ctx, const tpamd_field* f, const tpamd_field
                          const double* x, int64_t n, double t, int32_t precision,
                          double* out);
 int32_t tpamd_field_free(tpamd_ctx* ctx, tpamd_field* f);
+/* Fields and work rates at n states (host arrays xv[n][6], t[n]) on the
+ * device, in FP64: out[n][w] with w = 6 (E, B) when want_fields, + 4 (P_par,
+ * P_fermi, P_grad, P_betatron) when want_work, fields first.  The device
+ * counterpart of get_fields / get_work (src/boris/boris.jl:599-636,
+ * src/equations.jl:394-441, src/utility/utility.jl:379-422). */
+int32_t tpamd_record_eval(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, double q2m,
+                          double mass, const double* xv, const double* t, int64_t n,
+                          int32_t want_fields, int32_t want_work, double* out);
 
 /* ---- solve -------------------------------------------------------------- */
 /* _prepare_checks + _calculate_nout (src/boris/boris.jl:345-403): step count

@@ -52,6 +52,7 @@ class OrcDesc(C.Structure):
         ("nout_cap", C.c_long), ("r_in", C.c_double), ("r_out", C.c_double),
         ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
         ("dt_pp", C.POINTER(C.c_double)), ("nsteps_pp", C.c_long),
+        ("save_fields", C.c_int), ("save_work", C.c_int), ("mass", C.c_double),
     ]
 
 
@@ -76,6 +77,10 @@ def lib():
         L.orc_rational_polys.argtypes = [C.c_int, d3, d3, d3, d3, d3, C.POINTER(C.c_int)]
         L.orc_boris_tangent.argtypes = [C.POINTER(OrcDesc), C.POINTER(OrcField), d3, d3, d3]
         L.orc_field_eval.argtypes = [C.POINTER(OrcField), d3, C.c_double, d3]
+        L.orc_field_jac.argtypes = [C.POINTER(OrcField), d3, C.c_double, d3, d3, d3]
+        L.orc_work_rates.argtypes = [C.c_double, C.c_double, d3, d3, d3, d3, d3, d3]
+        L.orc_record_eval.argtypes = [C.POINTER(OrcField), C.POINTER(OrcField), C.c_double,
+                                      C.c_double, d3, d3, C.c_long, C.c_int, C.c_int, d3]
         L.orc_calculate_nout.argtypes = [C.c_long, C.c_long, C.c_int, C.c_int, C.c_int]
         L.orc_calculate_nout.restype = C.c_long
         L.orc_prepare_checks.argtypes = [C.POINTER(OrcDesc), C.POINTER(C.c_long),
@@ -217,7 +222,7 @@ def solve(xv0, E: Field, B: Field, *, q2m, tspan, dt=0.0, alg=ALG_BORIS, adaptiv
           save_start=True, save_end=True, save_everystep=True, maxiters=1_000_000,
           max_steps=0, nout_cap=None, bnd_flags=0, r_in=0.0, r_out=0.0,
           box_lo=(-np.inf,) * 3, box_hi=(np.inf,) * 3, dt_pp=None, nsteps_pp=0,
-          nthreads=1) -> OracleResult:
+          nthreads=1, save_fields=False, save_work=False, mass=0.0) -> OracleResult:
     xv0 = np.ascontiguousarray(np.atleast_2d(xv0), dtype=np.float64)
     N = xv0.shape[0]
     d = OrcDesc(alg=alg, adaptive=int(adaptive), order_N=order_N, nsub=nsub, form=form,
@@ -225,7 +230,9 @@ def solve(xv0, E: Field, B: Field, *, q2m, tspan, dt=0.0, alg=ALG_BORIS, adaptiv
                 save_everystep=int(save_everystep), bnd_flags=bnd_flags, safety=safety,
                 q2m=q2m, t0=tspan[0], t1=tspan[1], dt=dt, maxiters=maxiters,
                 savestepinterval=savestepinterval, max_steps=max_steps, r_in=r_in,
-                r_out=r_out, nsteps_pp=nsteps_pp)
+                r_out=r_out, nsteps_pp=nsteps_pp, save_fields=int(save_fields),
+                save_work=int(save_work), mass=mass)
+    width = 6 + (6 if save_fields else 0) + (4 if save_work else 0)
     d.box_lo[:] = [float(a) for a in box_lo]
     d.box_hi[:] = [float(a) for a in box_hi]
     dtpp = None
@@ -242,7 +249,7 @@ def solve(xv0, E: Field, B: Field, *, q2m, tspan, dt=0.0, alg=ALG_BORIS, adaptiv
             raise ValueError(ERR_NAMES.get(rc, str(rc)))
         nout_cap = max(int(nout.value), 1)
     d.nout_cap = nout_cap
-    out_u = np.full((N, nout_cap, 6), np.nan)
+    out_u = np.full((N, nout_cap, width), np.nan)
     out_t = np.full((N, nout_cap), np.nan)
     ln = np.zeros(N, dtype=np.int32)
     rc_arr = np.zeros(N, dtype=np.int32)
@@ -253,6 +260,33 @@ def solve(xv0, E: Field, B: Field, *, q2m, tspan, dt=0.0, alg=ALG_BORIS, adaptiv
     if rc:
         raise ValueError(ERR_NAMES.get(rc, str(rc)))
     return OracleResult(out_u, out_t, ln, rc_arr, float(el[0]))
+
+
+def field_jac(B: Field, x, t=0.0):
+    """B, its Jacobian J[i, j] = dB_i/dx_j and dB/dt at (x, t)."""
+    xa = np.ascontiguousarray(x, dtype=np.float64)
+    Bv, J, dBdt = np.zeros(3), np.zeros(9), np.zeros(3)
+    lib().orc_field_jac(C.byref(B.st), _dp(xa), float(t), _dp(Bv), _dp(J), _dp(dBdt))
+    return Bv, J.reshape(3, 3), dBdt
+
+
+def work_rates(q2m, m, v, E, B, J, dBdt):
+    """(P_par, P_fermi, P_grad, P_betatron) from the fields at one state."""
+    arr = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (v, E, B, J, dBdt)]
+    out = np.zeros(4)
+    lib().orc_work_rates(float(q2m), float(m), *[_dp(a) for a in arr], _dp(out))
+    return out
+
+
+def record_eval(E: Field, B: Field, *, q2m, m, xv, t, want_fields=True, want_work=True):
+    """The extra record columns at states xv (n, 6) and times t (n,)."""
+    xv = np.ascontiguousarray(np.atleast_2d(xv), dtype=np.float64)
+    t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, np.float64), (xv.shape[0],)))
+    w = (6 if want_fields else 0) + (4 if want_work else 0)
+    out = np.zeros((xv.shape[0], w))
+    lib().orc_record_eval(C.byref(E.st), C.byref(B.st), float(q2m), float(m), _dp(xv), _dp(t),
+                          xv.shape[0], int(want_fields), int(want_work), _dp(out))
+    return out
 
 
 def boris_tangent(xv0, B: Field, *, q2m, tspan, dt):

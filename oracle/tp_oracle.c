@@ -330,6 +330,162 @@ void orc_field_eval(const orc_field* f, const double x[3], double t, double out[
 }
 
 /* ----------------------------------------------------------------------------
+ * Saved-record augmentation, boris.jl:405-440.  The reference differentiates
+ * the field callables with ForwardDiff (utility.jl:386-396) and in time with
+ * derivative_t (prepare.jl:64-74); each kind has its derivative in closed form
+ * here.  Fields the restatement only uses for known-answer tests (KAT_*) get
+ * zero derivatives.
+ * ------------------------------------------------------------------------- */
+void orc_field_jac(const orc_field* f, const double x[3], double t, double B[3], double J[9],
+                   double dBdt[3])
+{
+    orc_field_eval(f, x, t, B);
+    for (int i = 0; i < 9; ++i) J[i] = 0.0;
+    dBdt[0] = dBdt[1] = dBdt[2] = 0.0;
+    switch (f->kind) {
+    case ORC_FIELD_AFFINE:
+        for (int i = 0; i < 9; ++i) J[i] = f->jac[i];
+        for (int i = 0; i < 3; ++i) dBdt[i] = f->ft[i];
+        return;
+    case ORC_FIELD_DIPOLE: {
+        /* B_i = k r^-5 g_i, g = (-3xz, -3yz, r^2 - 3z^2):
+         * dB_i/dx_j = k (r^-5 dg_i/dx_j - 5 r^-7 x_j g_i) */
+        const double r2 = sumsq3(x);
+        const double r = sqrt(r2);
+        const double r5 = (r2 * r2) * r;
+        const double k = f->b0 * ((f->radius * f->radius) * f->radius);
+        const double a = k / r5;
+        const double b = (5.0 * a) / r2;
+        const double g[3] = {(-3.0 * x[0]) * x[2], (-3.0 * x[1]) * x[2], r2 - (3.0 * x[2]) * x[2]};
+        const double dg[9] = {-3.0 * x[2], 0.0, -3.0 * x[0],
+                              0.0, -3.0 * x[2], -3.0 * x[1],
+                              2.0 * x[0], 2.0 * x[1], -4.0 * x[2]};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J[3 * i + j] = a * dg[3 * i + j] - (b * x[j]) * g[i];
+        return;
+    }
+    case ORC_FIELD_GRID: {
+        /* slope of the trilinear interpolant; zero along an axis where a
+         * clamped grid is clamped (the clamp's derivative) */
+        const long n[3] = {f->nx, f->ny, f->nz};
+        long i0[3], i1[3];
+        double w[3];
+        int flat[3];
+        for (int d = 0; d < 3; ++d) {
+            const double inv_h = 1.0 / f->spacing[d];
+            double s = (x[d] - f->origin[d]) * inv_h;
+            const double smax = (double)(n[d] - 1);
+            flat[d] = 0;
+            if (s != s || (f->extrap == ORC_EXTRAP_PERIODIC && !(fabs(s) < 4.0e18))) {
+                J[0] = NAN;
+                return;
+            }
+            if (f->extrap == ORC_EXTRAP_PERIODIC) {
+                const double fl = floor(s);
+                w[d] = s - fl;
+                long ii = (long)fl % n[d];
+                if (ii < 0) ii += n[d];
+                i0[d] = ii;
+                i1[d] = (ii + 1 == n[d]) ? 0 : ii + 1;
+            } else {
+                if (f->extrap == ORC_EXTRAP_FILL_NAN) {
+                    if (!(s >= 0.0 && s <= smax)) return;      /* B is NaN already */
+                } else {
+                    flat[d] = s < 0.0 || s > smax;
+                    s = s < 0.0 ? 0.0 : s;
+                    s = s > smax ? smax : s;
+                }
+                double fl = floor(s);
+                if (fl > smax - 1.0) fl = smax - 1.0;
+                w[d] = s - fl;
+                i0[d] = (long)fl;
+                i1[d] = i0[d] + 1;
+            }
+        }
+        for (int q = 0; q < 8; ++q) {
+            const long ii = (q & 1) ? i1[0] : i0[0], jj = ((q >> 1) & 1) ? i1[1] : i0[1],
+                       kk = (q >> 2) ? i1[2] : i0[2];
+            const double ax = (q & 1) ? w[0] : 1.0 - w[0];
+            const double ay = ((q >> 1) & 1) ? w[1] : 1.0 - w[1];
+            const double az = (q >> 2) ? w[2] : 1.0 - w[2];
+            const double sx = (q & 1) ? 1.0 : -1.0;
+            const double sy = ((q >> 1) & 1) ? 1.0 : -1.0;
+            const double sz = (q >> 2) ? 1.0 : -1.0;
+            const double dwx = (sx * ay) * az, dwy = (ax * sy) * az, dwz = (ax * ay) * sz;
+            for (int i = 0; i < 3; ++i) {
+                const double c = grid_node(f, ii, jj, kk, i);
+                J[3 * i] = J[3 * i] + c * dwx;
+                J[3 * i + 1] = J[3 * i + 1] + c * dwy;
+                J[3 * i + 2] = J[3 * i + 2] + c * dwz;
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            const double sd = flat[d] ? 0.0 : 1.0 / f->spacing[d];
+            for (int i = 0; i < 3; ++i) J[3 * i + d] = J[3 * i + d] * sd;
+        }
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+void orc_work_rates(double q2m, double m, const double v[3], const double E[3], const double B[3],
+                    const double J[9], const double dBdt[3], double out[4])
+{
+    const double q = q2m * m;                                    /* equations.jl:400 */
+    const double Bmag = sqrt(dot3(B, B));                        /* utility.jl:412 */
+    if (Bmag == 0.0) {                                           /* equations.jl:414-416 */
+        out[0] = out[1] = out[2] = out[3] = 0.0;
+        return;
+    }
+    double bh[3], gradB[3], Jb[3], kappa[3], vperp[3], bk[3], bg[3];
+    for (int i = 0; i < 3; ++i) bh[i] = B[i] / Bmag;             /* utility.jl:417 */
+    for (int j = 0; j < 3; ++j)                                  /* grad|B| = J' b, utility.jl:381 */
+        gradB[j] = (J[j] * bh[0] + J[3 + j] * bh[1]) + J[6 + j] * bh[2];
+    for (int i = 0; i < 3; ++i)
+        Jb[i] = (J[3 * i] * bh[0] + J[3 * i + 1] * bh[1]) + J[3 * i + 2] * bh[2];
+    const double gb = dot3(gradB, bh);
+    for (int i = 0; i < 3; ++i) kappa[i] = (Jb[i] + bh[i] * (-gb)) / Bmag;   /* utility.jl:383 */
+    const double vpar = dot3(v, bh);                             /* equations.jl:421-423 */
+    for (int i = 0; i < 3; ++i) vperp[i] = v[i] - vpar * bh[i];
+    const double wsq = dot3(vperp, vperp);                       /* equations.jl:426-427 */
+    const double mu = (m * wsq) / (2.0 * Bmag);
+    cross3(bh, kappa, bk);
+    cross3(bh, gradB, bg);
+    out[0] = (q * vpar) * dot3(E, bh);                           /* equations.jl:430 */
+    out[1] = ((m * (vpar * vpar)) / Bmag) * dot3(bk, E);        /* :433 */
+    out[2] = (mu / Bmag) * dot3(bg, E);                          /* :436 */
+    out[3] = mu * dot3(dBdt, bh);                                /* :439-441 */
+}
+
+/* the columns after x, v of one record, _prepare_saved_data (boris.jl:405-440) */
+static void record_columns(const orc_field* Ef, const orc_field* Bf, double q2m, double m,
+                           int want_fields, int want_work, const double x[3], const double v[3],
+                           double t, double* r)
+{
+    double E[3], B[3], J[9], dBdt[3];
+    orc_field_jac(Bf, x, t, B, J, dBdt);
+    orc_field_eval(Ef, x, t, E);
+    int c = 0;
+    if (want_fields) {
+        r[0] = E[0]; r[1] = E[1]; r[2] = E[2];
+        r[3] = B[0]; r[4] = B[1]; r[5] = B[2];
+        c = 6;
+    }
+    if (want_work) orc_work_rates(q2m, m, v, E, B, J, dBdt, r + c);
+}
+
+void orc_record_eval(const orc_field* E, const orc_field* B, double q2m, double m, const double* xv,
+                     const double* t, long n, int want_fields, int want_work, double* out)
+{
+    const int w = (want_fields ? 6 : 0) + (want_work ? 4 : 0);
+    for (long i = 0; i < n; ++i)
+        record_columns(E, B, q2m, m, want_fields, want_work, xv + 6 * i, xv + 6 * i + 3, t[i],
+                       out + w * i);
+}
+
+/* ----------------------------------------------------------------------------
  * Solve driver.
  * ------------------------------------------------------------------------- */
 typedef struct {
@@ -376,12 +532,16 @@ static int outside(const orc_desc* d, const double x[3], const double v[3])
     return 0;
 }
 
-static inline void record(double* ou, double* ot, long cap, long i, long k,
+static inline void record(const ctx_t* c, double* ou, double* ot, long cap, long i, long k,
                           const double x[3], const double v[3], double t)
 {
-    double* r = ou + (i * cap + k) * 6;
+    const int w = ORC_RECORD_WIDTH(c->d);
+    double* r = ou + (i * cap + k) * w;
     r[0] = x[0]; r[1] = x[1]; r[2] = x[2];
     r[3] = v[0]; r[4] = v[1]; r[5] = v[2];
+    if (w > 6)
+        record_columns(c->E, c->B, c->d->q2m, c->d->mass, c->d->save_fields, c->d->save_work, x, v, t,
+                       r + 6);
     ot[i * cap + k] = t;
 }
 
@@ -438,7 +598,7 @@ static void solve_fixed_one(const ctx_t* c, long i, const double* xv0, long nt, 
     double tlast = 0.0;
 
     if (d->save_start) {                                        /* :507-511 */
-        record(ou, ot, cap, i, iout, x, v, t0);
+        record(c, ou, ot, cap, i, iout, x, v, t0);
         tlast = t0;
         iout += 1;
     }
@@ -466,7 +626,7 @@ static void solve_fixed_one(const ctx_t* c, long i, const double* xv0, long nt, 
             if (iout <= nout) {
                 const double tc = fma(-0.5, dt, t);             /* :466 @muladd scalar */
                 updater(c, vprev, x, 0.5 * dt, tc, vtmp);       /* :467 */
-                record(ou, ot, cap, i, iout - 1, x, vtmp, tc);
+                record(c, ou, ot, cap, i, iout - 1, x, vtmp, tc);
                 tlast = tc;
             }
         }
@@ -484,7 +644,7 @@ loop_done:;
         if (iout == 0 || tlast < tf) {
             const double dtf = tf - fma((double)final_step - 0.5, dt, t0);   /* :534 */
             updater(c, v, x, dtf, tf, vtmp);                                /* :535 */
-            record(ou, ot, cap, i, iout, x, vtmp, tf);
+            record(c, ou, ot, cap, i, iout, x, vtmp, tf);
             iout += 1;
         }
     }
@@ -512,7 +672,7 @@ static void solve_adaptive_one(const ctx_t* c, long i, const double* xv0, double
 
 #define PUSH(xx, vv, tt)                                         \
     do {                                                         \
-        if (nrec < cap) record(ou, ot, cap, i, nrec, xx, vv, tt); \
+        if (nrec < cap) record(c, ou, ot, cap, i, nrec, xx, vv, tt); \
         tlast = (tt);                                            \
         nrec += 1;                                               \
     } while (0)

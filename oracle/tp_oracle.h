@@ -83,7 +83,13 @@ typedef struct orc_desc {
     double box_lo[3], box_hi[3];
     const double* dt_pp;   /* per-particle dt (fixed step), NULL = use dt */
     long nsteps_pp;        /* steps per particle when dt_pp != NULL */
+    int save_fields;       /* records gain E, B (boris.jl:405-440) */
+    int save_work;         /* records gain P_par, P_fermi, P_grad, P_betatron */
+    double mass;           /* p[2], for the work rates */
 } orc_desc;
+
+/* record columns: 6 + 6 (save_fields) + 4 (save_work) */
+#define ORC_RECORD_WIDTH(d) (6 + ((d)->save_fields ? 6 : 0) + ((d)->save_work ? 4 : 0))
 
 /* boris.jl:19-31 */
 void orc_boris_velocity_update(const double v[3], const double E[3],
@@ -98,6 +104,18 @@ void orc_multistep_coeffs(double tau2, int n, int form, double c[4]);
 int orc_rational_polys(int n, double* D, double* P1, double* P2, double* P3,
                        double* P6, int* deg);
 void orc_field_eval(const orc_field* f, const double x[3], double t, double out[3]);
+/* B, J[3*i+j] = dB_i/dx_j and dB/dt at (x, t): the derivatives the reference
+ * takes with ForwardDiff (utility.jl:386-396, prepare.jl:64-74), in closed form */
+void orc_field_jac(const orc_field* f, const double x[3], double t, double B[3], double J[9],
+                   double dBdt[3]);
+/* get_work_rates, equations.jl:394-441, with get_magnetic_properties and
+ * _get_curvature (utility.jl:379-422): out = P_par, P_fermi, P_grad, P_betatron */
+void orc_work_rates(double q2m, double m, const double v[3], const double E[3], const double B[3],
+                    const double J[9], const double dBdt[3], double out[4]);
+/* the extra record columns at n states: E, B when want_fields, then the work
+ * rates when want_work (get_fields / get_work, boris.jl:599-636) */
+void orc_record_eval(const orc_field* E, const orc_field* B, double q2m, double m, const double* xv,
+                     const double* t, long n, int want_fields, int want_work, double* out);
 /* boris.jl:383-403 */
 long orc_calculate_nout(long nt, long K, int save_start, int save_end, int save_everystep);
 /* boris.jl:345-362 */

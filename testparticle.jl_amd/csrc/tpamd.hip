@@ -46,14 +46,14 @@ struct S64 {
     double v[3];
     double dt_cache;
 
-    __device__ void load(const KParams& p, long long i)
+    __device__ __forceinline__ void load(const KParams& p, long long i)
     {
         const double* s = p.xv0 + 6 * i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) { pos.x[c] = s[c]; v[c] = s[3 + c]; }
     }
-    __device__ void set_dt(double dt) { dt_cache = dt; }
-    __device__ void fields(const KParams& p, double t, double E[3], double B[3]) const
+    __device__ __forceinline__ void set_dt(double dt) { dt_cache = dt; }
+    __device__ __forceinline__ void fields(const KParams& p, double t, double E[3], double B[3]) const
     {
         field_f64(p.B, pos.x, t, B);
         if (p.e_is_uxb) cross3(B, p.E.v, E);       // E = -u x B = B x u
@@ -62,35 +62,35 @@ struct S64 {
     // fields at the current position, then the velocity update from given
     // fields: update_velocity (boris.jl:38-45) / MultistepUpdater
     // (multistep_boris.jl:7-9) is U = eval + Uf
-    __device__ void eval(const KParams& p, double t, double E[3], double B[3]) const
+    __device__ __forceinline__ void eval(const KParams& p, double t, double E[3], double B[3]) const
     {
         fields(p, t, E, B);
     }
     template <int ALG>
-    __device__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
+    __device__ __forceinline__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
                        double h, double out[3]) const
     {
         if (ALG == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
         else multistep_f64(p, vin, E, B, h, out);
     }
     template <int ALG>
-    __device__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
+    __device__ __forceinline__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
     {
         double E[3], B[3];
         eval(p, t, E, B);
         Uf<ALG>(p, E, B, vin, h, out);
     }
-    __device__ double bmag_of(const double B[3]) const { return sqrt(dot3(B, B)); }
-    __device__ void step(const double vv[3], double dt, Pos& xn) const
+    __device__ __forceinline__ double bmag_of(const double B[3]) const { return sqrt(dot3(B, B)); }
+    __device__ __forceinline__ void step(const double vv[3], double dt, Pos& xn) const
     {
 #pragma unroll
         for (int c = 0; c < 3; ++c) xn.x[c] = pos.x[c] + vv[c] * dt;
     }
-    __device__ void abs_pos(const KParams&, const Pos& q, double o[3]) const
+    __device__ __forceinline__ void abs_pos(const KParams&, const Pos& q, double o[3]) const
     {
         o[0] = q.x[0]; o[1] = q.x[1]; o[2] = q.x[2];
     }
-    __device__ void vel_d(const double vv[3], double o[3]) const
+    __device__ __forceinline__ void vel_d(const double vv[3], double o[3]) const
     {
         o[0] = vv[0]; o[1] = vv[1]; o[2] = vv[2];
     }
@@ -104,8 +104,8 @@ struct S32C {
     float v[3];
     float dt_f, dt_hi, dt_lo;
 
-    __device__ void load(const KParams& p, long long i) { load_from(p, p.xv0 + 6 * i); }
-    __device__ void load_from(const KParams& p, const double* s)
+    __device__ __forceinline__ void load(const KParams& p, long long i) { load_from(p, p.xv0 + 6 * i); }
+    __device__ __forceinline__ void load_from(const KParams& p, const double* s)
     {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -122,13 +122,13 @@ struct S32C {
             v[c] = (float)s[3 + c];
         }
     }
-    __device__ void set_dt(double dt)
+    __device__ __forceinline__ void set_dt(double dt)
     {
         dt_hi = (float)dt;
         dt_lo = (float)(dt - (double)dt_hi);
         dt_f = dt_hi;
     }
-    __device__ void fields(const KParams& p, const Pos& q, double t, float E[3], float B[3]) const
+    __device__ __forceinline__ void fields(const KParams& p, const Pos& q, double t, float E[3], float B[3]) const
     {
         field_f32(p, p.B, q, t, B);
         if (p.e_is_uxb) {
@@ -138,26 +138,26 @@ struct S32C {
             field_f32(p, p.E, q, t, E);
         }
     }
-    __device__ void eval(const KParams& p, double t, float E[3], float B[3]) const
+    __device__ __forceinline__ void eval(const KParams& p, double t, float E[3], float B[3]) const
     {
         fields(p, pos, t, E, B);
     }
     template <int ALG>
-    __device__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
+    __device__ __forceinline__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
                        double h, float out[3]) const
     {
         if (ALG == 0) boris_f32(vin, E, B, (float)((p.q2m * 0.5) * h), out);
         else multistep_f32(p, vin, E, B, (float)h, out);
     }
     template <int ALG>
-    __device__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
+    __device__ __forceinline__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
     {
         float E[3], B[3];
         eval(p, t, E, B);
         Uf<ALG>(p, E, B, vin, h, out);
     }
-    __device__ double bmag_of(const float B[3]) const { return (double)sqrtf(dot3f(B, B)); }
-    __device__ void step_impl(const KParams& p, const float vv[3], Pos& xn) const
+    __device__ __forceinline__ double bmag_of(const float B[3]) const { return (double)sqrtf(dot3f(B, B)); }
+    __device__ __forceinline__ void step_impl(const KParams& p, const float vv[3], Pos& xn) const
     {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -181,8 +181,8 @@ struct S32C {
             xn.wrap[c] = w;
         }
     }
-    __device__ void abs_pos(const KParams& p, const Pos& q, double o[3]) const { pos_abs(p, q, o); }
-    __device__ void vel_d(const float vv[3], double o[3]) const
+    __device__ __forceinline__ void abs_pos(const KParams& p, const Pos& q, double o[3]) const { pos_abs(p, q, o); }
+    __device__ __forceinline__ void vel_d(const float vv[3], double o[3]) const
     {
         o[0] = (double)vv[0]; o[1] = (double)vv[1]; o[2] = (double)vv[2];
     }
@@ -223,16 +223,42 @@ __device__ __forceinline__ bool outside(const KParams& p, const S& s, const type
     return false;
 }
 
-template <class S>
+// extra record columns (E, B and/or work rates) at (x, v, t), in FP64.  Only
+// the REC instantiations of the generic kernels contain it, so the pushers
+// that save plain records keep their register budget.
+__device__ __forceinline__ void record_extra(const KParams& p, const double x[3], const double v[3],
+                                          double t, double* r)
+{
+    double E[3], B[3], J[3][3], dBdt[3];
+    field_jac_f64(p.B, x, t, B, J, dBdt);
+    if (p.e_is_uxb) cross3(B, p.E.v, E);                  // E = -u x B = B x u
+    else field_f64(p.E, x, t, E);
+    int c = 6;
+    if (p.save_fields) {
+        r[6] = E[0]; r[7] = E[1]; r[8] = E[2];
+        r[9] = B[0]; r[10] = B[1]; r[11] = B[2];
+        c = 12;
+    }
+    if (p.save_work) {
+        double w[4];
+        work_rates_f64(p.q2m, p.mass, v, E, B, J, dBdt, w);
+        r[c] = w[0]; r[c + 1] = w[1]; r[c + 2] = w[2]; r[c + 3] = w[3];
+    }
+}
+
+// one record (_prepare_saved_data, boris.jl:405-440): x, v, then E and B,
+// then the work rates, as the descriptor asks (REC kernels only)
+template <bool REC, class S>
 __device__ __forceinline__ void record(const KParams& p, const S& s, long long i, long long k,
                                        const typename S::R vv[3], double t)
 {
     double x[3], v[3];
     s.abs_pos(p, s.pos, x);
     s.vel_d(vv, v);
-    double* r = p.out_u + (i * p.cap + k) * 6;
+    double* r = p.out_u + (i * p.cap + k) * p.rec_w;
     r[0] = x[0]; r[1] = x[1]; r[2] = x[2];
     r[3] = v[0]; r[4] = v[1]; r[5] = v[2];
+    if constexpr (REC) record_extra(p, x, v, t, r);
     p.out_t[i * p.cap + k] = t;
 }
 
@@ -246,8 +272,8 @@ __device__ __forceinline__ void do_step(const KParams& p, const S& s, const type
 
 // _boris_single + _boris_loop!, src/boris/boris.jl:445-557.  The save test
 // `(it - 1) % K == 0` is carried as a countdown to the next save step.
-template <class S, int ALG>
-__device__ long long run_fixed(const KParams& p, long long i)
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
 {
     using R = typename S::R;
     S s;
@@ -259,7 +285,7 @@ __device__ long long run_fixed(const KParams& p, long long i)
     long long iout = 0;
     double tlast = 0.0;
     if (p.save_start) {                                       // :507-511
-        record(p, s, i, iout, s.v, t0);
+        record<REC>(p, s, i, iout, s.v, t0);
         tlast = t0;
         iout += 1;
     }
@@ -296,7 +322,7 @@ __device__ long long run_fixed(const KParams& p, long long i)
                 // same position as the step's evaluation: static fields are re-used
                 if (p.static_fields) s.template Uf<ALG>(p, Ec, Bc, vprev, 0.5 * dt, vs);
                 else s.template U<ALG>(p, vprev, 0.5 * dt, tc, vs);
-                record(p, s, i, iout - 1, vs, tc);
+                record<REC>(p, s, i, iout - 1, vs, tc);
                 tlast = tc;
             }
         }
@@ -311,7 +337,7 @@ __device__ long long run_fixed(const KParams& p, long long i)
             const double dtf = tf - fma((double)final_step - 0.5, dt, t0);
             R vf[3];
             s.template U<ALG>(p, s.v, dtf, tf, vf);
-            record(p, s, i, iout, vf, tf);
+            record<REC>(p, s, i, iout, vf, tf);
             iout += 1;
         }
     }
@@ -321,8 +347,8 @@ __device__ long long run_fixed(const KParams& p, long long i)
 }
 
 // _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266
-template <class S, int ALG>
-__device__ long long run_adaptive(const KParams& p, long long i)
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
 {
     using R = typename S::R;
     S s;
@@ -335,7 +361,7 @@ __device__ long long run_adaptive(const KParams& p, long long i)
     double tlast = 0.0;
     R vt[3];
     auto push = [&](const R vv[3], double tt) {
-        if (nrec < p.cap) record(p, s, i, nrec, vv, tt);
+        if (nrec < p.cap) record<REC>(p, s, i, nrec, vv, tt);
         tlast = tt;
         nrec += 1;
     };
@@ -422,21 +448,21 @@ __device__ __forceinline__ void add_steps(const KParams& p, long long steps)
     if ((threadIdx.x & 63) == 0) atomicAdd(p.steps, v);
 }
 
-template <class S, int ALG>
+template <class S, int ALG, bool REC = false>
 __global__ void __launch_bounds__(256) k_fixed(const KParams p)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long steps = 0;
-    if (i < p.N) steps = run_fixed<S, ALG>(p, i);
+    if (i < p.N) steps = run_fixed<S, ALG, REC>(p, i);
     add_steps(p, steps);
 }
 
-template <class S, int ALG>
+template <class S, int ALG, bool REC = false>
 __global__ void __launch_bounds__(256) k_adaptive(const KParams p)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long steps = 0;
-    if (i < p.N) steps = run_adaptive<S, ALG>(p, i);
+    if (i < p.N) steps = run_adaptive<S, ALG, REC>(p, i);
     add_steps(p, steps);
 }
 
@@ -447,7 +473,7 @@ __global__ void __launch_bounds__(256) k_adaptive(const KParams p)
 template <int EMODE, int NS, bool PER, typename T, bool CELLS>
 struct S64G : S64 {
     static constexpr int kNS = NS;
-    __device__ void bfield(const KParams& p, double B[3]) const
+    __device__ __forceinline__ void bfield(const KParams& p, double B[3]) const
     {
         int i0[3], i1[3];
         double w[3];
@@ -467,21 +493,21 @@ struct S64G : S64 {
             trilinear<T, double>(p.B.data, ox, oy, oz, w, B);
         }
     }
-    __device__ void eval(const KParams& p, double, double E[3], double B[3]) const
+    __device__ __forceinline__ void eval(const KParams& p, double, double E[3], double B[3]) const
     {
         bfield(p, B);
         if (EMODE == kEUxB) cross3(B, p.E.v, E);
         else { E[0] = p.E.v[0]; E[1] = p.E.v[1]; E[2] = p.E.v[2]; }
     }
     template <int ALG>
-    __device__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
+    __device__ __forceinline__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
                        double h, double out[3]) const
     {
         if (NS == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
         else multistep_f64_n<NS == 0 ? 1 : NS>(p, vin, E, B, h, out);
     }
     template <int ALG>
-    __device__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
+    __device__ __forceinline__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
     {
         double E[3], B[3];
         eval(p, t, E, B);
@@ -492,7 +518,7 @@ struct S64G : S64 {
 template <int EMODE, int NS, bool PER, bool CELLS>
 struct S32G : S32C {
     static constexpr int kNS = NS;
-    __device__ void bfield(const KParams& p, float B[3]) const
+    __device__ __forceinline__ void bfield(const KParams& p, float B[3]) const
     {
         int i0[3], i1[3];
         float w[3];
@@ -512,21 +538,21 @@ struct S32G : S32C {
             trilinear<float, float>(p.B.data, ox, oy, oz, w, B);
         }
     }
-    __device__ void eval(const KParams& p, double, float E[3], float B[3]) const
+    __device__ __forceinline__ void eval(const KParams& p, double, float E[3], float B[3]) const
     {
         bfield(p, B);
         if (EMODE == kEUxB) cross3f(B, p.f32.u, E);
         else { E[0] = p.f32.e0[0]; E[1] = p.f32.e0[1]; E[2] = p.f32.e0[2]; }
     }
     template <int ALG>
-    __device__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
+    __device__ __forceinline__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
                        double h, float out[3]) const
     {
         if (NS == 0) boris_f32_fast(vin, E, B, (float)((p.q2m * 0.5) * h), out);
         else multistep_f32_n<NS == 0 ? 1 : NS>(p, vin, E, B, (float)(p.q2m_2n * h), out);
     }
     template <int ALG>
-    __device__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
+    __device__ __forceinline__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
     {
         float E[3], B[3];
         eval(p, t, E, B);
@@ -836,7 +862,7 @@ __global__ void k_batch_reset(unsigned* counter, unsigned long long* steps)
     }
 }
 
-template <class S, int ALG, bool ADAPT>
+template <class S, int ALG, bool ADAPT, bool REC = false>
 __global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table,
                                                const Chunk* __restrict__ chunks, int nchunks,
                                                unsigned* __restrict__ counter,
@@ -857,8 +883,8 @@ __global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table
         const Chunk ch = chunks[c];
         const KParams& p = table[ch.inst];
         for (long long i = ch.begin + lane; i < ch.end; i += 64) {
-            if constexpr (ADAPT) steps += run_adaptive<S, ALG>(p, i);
-            else steps += run_fixed<S, ALG>(p, i);
+            if constexpr (ADAPT) steps += run_adaptive<S, ALG, REC>(p, i);
+            else steps += run_fixed<S, ALG, REC>(p, i);
         }
         c = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
     }
@@ -891,6 +917,24 @@ __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const doubl
         const float* o = which ? E : B;
         for (int c = 0; c < 3; ++c) out[3 * i + c] = (double)o[c];
     }
+}
+
+// fields and work rates at given states: the columns record_extra writes,
+// without the solve (p.save_fields / p.save_work / p.rec_w select them)
+__global__ void __launch_bounds__(256) k_record_eval(const KParams p, const double* __restrict__ xv,
+                                                     const double* __restrict__ t, long long n,
+                                                     double* __restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x[3], v[3], r[16];
+    for (int c = 0; c < 3; ++c) {
+        x[c] = xv[6 * i + c];
+        v[c] = xv[6 * i + 3 + c];
+    }
+    record_extra(p, x, v, t[i], r);
+    const int w = p.rec_w - 6;
+    for (int c = 0; c < w; ++c) out[(size_t)w * i + c] = r[6 + c];
 }
 
 }  // namespace
@@ -1106,6 +1150,7 @@ void sel_g64_ns(const L& l, int ns, bool per)
 int spec_class(const tpamd_solve_desc* d, const KParams& p, int* emode)
 {
     if (d->flags & TPAMD_FLAG_GENERIC_KERNELS) return -1;
+    if (TPAMD_RECORD_WIDTH(d) > 6) return -1;       // E, B, work columns: generic REC kernels
     if (p.B.kind != kFieldGrid) return -1;
     if (p.e_is_uxb) *emode = kEUxB;
     else if (p.E.kind == kFieldZero || p.E.kind == kFieldUniform) *emode = kEUniform;
@@ -1168,6 +1213,15 @@ struct SoloLaunch {
     template <class S, int ALG>
     void go() const
     {
+        // records with E, B or work columns: generic policies only (spec_class
+        // declines them), in the REC instantiations
+        if constexpr (S::kNS < 0) {
+            if (p.rec_w > 6) {
+                if (adaptive) hipLaunchKernelGGL((k_adaptive<S, ALG, true>), g, b, 0, st, p);
+                else hipLaunchKernelGGL((k_fixed<S, ALG, true>), g, b, 0, st, p);
+                return;
+            }
+        }
         if constexpr (S::kNS <= 0) {
             if (adaptive) {
                 hipLaunchKernelGGL((k_adaptive<S, ALG>), g, b, 0, st, p);
@@ -1190,6 +1244,7 @@ struct BatchLaunch {
     dim3 g, b;
     hipStream_t st;
     int* occ;
+    bool rec;                 // records carry E, B or work columns
     template <class K>
     void run(K kern) const
     {
@@ -1203,6 +1258,13 @@ struct BatchLaunch {
     template <class S, int ALG>
     void go() const
     {
+        if constexpr (S::kNS < 0) {
+            if (rec) {
+                if (adaptive) run(k_batch<S, ALG, true, true>);
+                else run(k_batch<S, ALG, false, true>);
+                return;
+            }
+        }
         if constexpr (S::kNS <= 0) {
             if (adaptive) {
                 run(k_batch<S, ALG, true>);
@@ -1271,6 +1333,10 @@ int fill_params(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field*
     p.nout = nout;
     p.cap = desc->nout_cap;
     p.K = desc->savestepinterval;
+    p.save_fields = desc->save_fields != 0;
+    p.save_work = desc->save_work != 0;
+    p.rec_w = TPAMD_RECORD_WIDTH(desc);
+    p.mass = desc->mass;
     p.max_steps = desc->max_steps;
     p.N = N;
     p.r_in = desc->r_in;
@@ -1407,8 +1473,9 @@ int batch_enqueue(tpamd_batch* b, hipStream_t st)
     tpamd_ctx* ctx = b->ctx;
     const dim3 block(b->block_size);
     int occ = 0;
+    const bool rec = TPAMD_RECORD_WIDTH(&b->desc) > 6;
     BatchLaunch q{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
-                  dim3(1), block, st, &occ};
+                  dim3(1), block, st, &occ, rec};
     select_policy(&b->desc, b->table[0], b->allow_spec, q);
     occ = std::max(occ, 1);
     const long long waves_needed = b->nchunks;
@@ -1422,7 +1489,7 @@ int batch_enqueue(tpamd_batch* b, hipStream_t st)
     blocks = std::max(blocks, 1LL);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counter, b->d_steps);
     BatchLaunch l{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
-                  dim3((unsigned)blocks), block, st, nullptr};
+                  dim3((unsigned)blocks), block, st, nullptr, rec};
     select_policy(&b->desc, b->table[0], b->allow_spec, l);
     HIP_TRY(ctx, hipGetLastError());
     return TPAMD_OK;
@@ -1860,6 +1927,44 @@ int32_t tpamd_field_eval(tpamd_ctx* ctx, const tpamd_field* f, const tpamd_field
     return TPAMD_OK;
 }
 
+int32_t tpamd_record_eval(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, double q2m,
+                          double mass, const double* xv, const double* t, int64_t n,
+                          int32_t want_fields, int32_t want_work, double* out)
+{
+    if (!ctx || !E || !B || n < 0 || (n > 0 && (!xv || !t || !out)))
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
+    if (!want_fields && !want_work) return fail(ctx, TPAMD_ERR_ARGUMENT, "nothing asked for");
+    KParams p;
+    std::memset(&p, 0, sizeof p);
+    int rc = setup_fields(ctx, E, B, TPAMD_FP64, p);
+    if (rc) return rc;
+    p.q2m = q2m;
+    p.mass = mass;
+    p.save_fields = want_fields != 0;
+    p.save_work = want_work != 0;
+    p.rec_w = 6 + (p.save_fields ? 6 : 0) + (p.save_work ? 4 : 0);
+    if (n == 0) return TPAMD_OK;
+    const int w = p.rec_w - 6;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double* dxv = nullptr;
+    HIP_TRY(ctx, hipMalloc((void**)&dxv, sizeof(double) * (7 + (size_t)w) * n));
+    double* dt = dxv + 6 * n;
+    double* dout = dt + n;
+    hipError_t e = hipMemcpyAsync(dxv, xv, sizeof(double) * 6 * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, t, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_record_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p,
+                           dxv, dt, (long long)n, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, sizeof(double) * w * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dxv);
+    if (e != hipSuccess) return fail(ctx, TPAMD_ERR_DEVICE, "record eval failed: %s", hipGetErrorString(e));
+    return TPAMD_OK;
+}
+
 int32_t tpamd_plan(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int64_t* nt, int64_t* nout)
 {
     return check_desc(ctx, desc, nt, nout);
@@ -1896,7 +2001,7 @@ int32_t tpamd_solve(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_fi
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t cap = desc->nout_cap;
     const size_t b_xv = sizeof(double) * 6 * (size_t)N;
-    const size_t b_u = sizeof(double) * 6 * (size_t)cap * (size_t)N;
+    const size_t b_u = sizeof(double) * (size_t)TPAMD_RECORD_WIDTH(desc) * (size_t)cap * (size_t)N;
     const size_t b_t = sizeof(double) * (size_t)cap * (size_t)N;
     const size_t b_i = sizeof(int32_t) * (size_t)N;
     const size_t b_dt = desc->dt_pp ? sizeof(double) * (size_t)N : 0;
@@ -2173,7 +2278,7 @@ int32_t tpamd_solve_batch(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t 
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t cap = desc->nout_cap;
     const size_t b_xv = sizeof(double) * 6 * (size_t)N;
-    const size_t b_u = sizeof(double) * 6 * (size_t)cap * (size_t)N;
+    const size_t b_u = sizeof(double) * (size_t)TPAMD_RECORD_WIDTH(desc) * (size_t)cap * (size_t)N;
     const size_t b_t = sizeof(double) * (size_t)cap * (size_t)N;
     const size_t b_i = sizeof(int32_t) * (size_t)N;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };

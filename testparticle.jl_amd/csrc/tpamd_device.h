@@ -82,7 +82,10 @@ struct KParams {
     int save_start, save_end, save_every, bnd_flags;
     int e_is_uxb, frame_periodic;
     int static_fields;        // neither E nor B depends on t
+    int rec_w;                // record columns: 6, + 6 (E, B), + 4 (work rates)
+    int save_fields, save_work;
     int pad1;
+    double mass;              // for the work rates
     double q2m, t0, t1, dt, C;
     long long nt, nout, cap, K, max_steps, N;
     double r_in, r_out;
@@ -142,10 +145,19 @@ __device__ __forceinline__ void boris_f64(const double v[3], const double E[3], 
     for (int c = 0; c < 3; ++c) out[c] = (vm[c] + vps[c]) + h * E[c];
 }
 
+// Horner over p[0..d], d <= NMAX: r = p[d], then r = fma(r, u, p[j]) for
+// j = d-1 .. 0.  The loop runs NMAX+1 unrolled iterations with static
+// indices (a runtime index into the kernel-argument block would make the
+// compiler copy the block to scratch memory).
+template <int NMAX>
 __device__ __forceinline__ double horner_f64(const double* p, int d, double u)
 {
-    double r = p[d];
-    for (int j = d - 1; j >= 0; --j) r = fma(r, u, p[j]);
+    double r = 0.0;
+#pragma unroll
+    for (int j = NMAX; j >= 0; --j) {
+        if (j == d) r = p[j];
+        else if (j < d) r = fma(r, u, p[j]);
+    }
     return r;
 }
 
@@ -179,11 +191,11 @@ __device__ __forceinline__ void multistep_f64(const KParams& p, const double v[3
     const double tau2 = dot3(tn, tn);
     double c1, c2, c3, c6;
     if (p.form == 1) {
-        const double rD = 1.0 / horner_f64(p.poly.D, p.poly.deg, tau2);
-        c1 = horner_f64(p.poly.P1, p.poly.deg, tau2) * rD;
-        c2 = horner_f64(p.poly.P2, p.poly.deg - 1, tau2) * rD;
-        c3 = horner_f64(p.poly.P3, p.poly.deg - 1, tau2) * rD;
-        c6 = horner_f64(p.poly.P6, p.poly.deg - 1, tau2) * rD;
+        const double rD = 1.0 / horner_f64<kRationalNMax>(p.poly.D, p.poly.deg, tau2);
+        c1 = horner_f64<kRationalNMax>(p.poly.P1, p.poly.deg, tau2) * rD;
+        c2 = horner_f64<kRationalNMax - 1>(p.poly.P2, p.poly.deg - 1, tau2) * rD;
+        c3 = horner_f64<kRationalNMax - 1>(p.poly.P3, p.poly.deg - 1, tau2) * rD;
+        c6 = horner_f64<kRationalNMax - 1>(p.poly.P6, p.poly.deg - 1, tau2) * rD;
     } else {
         const double tmag = sqrt(tau2);
         const double nn = (double)n;
@@ -337,6 +349,158 @@ __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], 
 }
 
 // ---------------------------------------------------------------------------
+// Saved-record augmentation (src/boris/boris.jl:405-440): B, its Jacobian
+// J[i][j] = dB_i/dx_j and dB/dt at (x, t), and the work rates
+// (src/equations.jl:394-441, src/utility/utility.jl:379-422).  The reference
+// differentiates the field callables with ForwardDiff; here each field kind
+// has its derivative in closed form.  B itself is field_f64's value.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void field_jac_f64(const DevField& f, const double x[3], double t,
+                                              double B[3], double J[3][3], double dBdt[3])
+{
+    field_f64(f, x, t, B);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        dBdt[i] = 0.0;
+        J[i][0] = J[i][1] = J[i][2] = 0.0;
+    }
+    switch (f.kind) {
+    case kFieldAffine:
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            J[i][0] = f.jac[3 * i];
+            J[i][1] = f.jac[3 * i + 1];
+            J[i][2] = f.jac[3 * i + 2];
+            dBdt[i] = f.ft[i];
+        }
+        return;
+    case kFieldDipole: {
+        // B_i = k r^-5 g_i, g = (-3xz, -3yz, r^2 - 3z^2):
+        // dB_i/dx_j = k (r^-5 dg_i/dx_j - 5 r^-7 x_j g_i)
+        const double r2 = dot3(x, x);
+        const double r = sqrt(r2);
+        const double r5 = (r2 * r2) * r;
+        const double k = f.b0 * ((f.radius * f.radius) * f.radius);
+        const double a = k / r5;
+        const double b = (5.0 * a) / r2;
+        const double g[3] = {(-3.0 * x[0]) * x[2], (-3.0 * x[1]) * x[2], r2 - (3.0 * x[2]) * x[2]};
+        const double dg[3][3] = {{-3.0 * x[2], 0.0, -3.0 * x[0]},
+                                 {0.0, -3.0 * x[2], -3.0 * x[1]},
+                                 {2.0 * x[0], 2.0 * x[1], -4.0 * x[2]}};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) J[i][j] = a * dg[i][j] - (b * x[j]) * g[i];
+        return;
+    }
+    case kFieldGrid: {
+        // trilinear slope; zero along an axis where a clamped grid is clamped
+        int i0[3], i1[3];
+        double w[3];
+        bool flat[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int nd = f.n[d];
+            double s = (x[d] - f.origin[d]) * f.inv_h[d];
+            const double smax = (double)(nd - 1);
+            flat[d] = false;
+            if (s != s || (f.extrap == kExtrapPeriodic && !(fabs(s) < 4.0e18))) {
+                J[0][0] = __builtin_nan("");
+                return;
+            }
+            if (f.extrap == kExtrapPeriodic) {
+                const double fl = floor(s);
+                w[d] = s - fl;
+                long long ii = (long long)fl % (long long)nd;
+                if (ii < 0) ii += nd;
+                i0[d] = (int)ii;
+                i1[d] = (i0[d] + 1 == nd) ? 0 : i0[d] + 1;
+            } else {
+                if (f.extrap == kExtrapFillNaN) {
+                    if (!(s >= 0.0 && s <= smax)) return;     // B is NaN already
+                } else {
+                    flat[d] = s < 0.0 || s > smax;
+                    s = s < 0.0 ? 0.0 : s;
+                    s = s > smax ? smax : s;
+                }
+                double fl = floor(s);
+                if (fl > smax - 1.0) fl = smax - 1.0;
+                w[d] = s - fl;
+                i0[d] = (int)fl;
+                i1[d] = i0[d] + 1;
+            }
+        }
+        const long long nx = f.n[0], nxy = (long long)f.n[0] * f.n[1];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double c[3];
+            node3_f64(f, ((q & 1) ? i1[0] : i0[0]) + nx * (((q >> 1) & 1) ? i1[1] : i0[1]) +
+                             nxy * ((q >> 2) ? i1[2] : i0[2]), c);
+            const double ax = (q & 1) ? w[0] : 1.0 - w[0];
+            const double ay = ((q >> 1) & 1) ? w[1] : 1.0 - w[1];
+            const double az = (q >> 2) ? w[2] : 1.0 - w[2];
+            const double sx = (q & 1) ? 1.0 : -1.0;
+            const double sy = ((q >> 1) & 1) ? 1.0 : -1.0;
+            const double sz = (q >> 2) ? 1.0 : -1.0;
+            const double dwx = (sx * ay) * az, dwy = (ax * sy) * az, dwz = (ax * ay) * sz;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                J[i][0] = J[i][0] + c[i] * dwx;
+                J[i][1] = J[i][1] + c[i] * dwy;
+                J[i][2] = J[i][2] + c[i] * dwz;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double sd = flat[d] ? 0.0 : f.inv_h[d];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) J[i][d] = J[i][d] * sd;
+        }
+        return;
+    }
+    default:  // zero, uniform
+        return;
+    }
+}
+
+// get_work_rates (src/equations.jl:394-441) with get_magnetic_properties and
+// _get_curvature (src/utility/utility.jl:379-422): out = (P_par, P_fermi,
+// P_grad, P_betatron)
+__device__ __forceinline__ void work_rates_f64(double q2m, double m, const double v[3],
+                                               const double E[3], const double B[3],
+                                               const double J[3][3], const double dBdt[3],
+                                               double out[4])
+{
+    const double q = q2m * m;
+    const double Bmag = sqrt(dot3(B, B));
+    if (Bmag == 0.0) {
+        out[0] = out[1] = out[2] = out[3] = 0.0;
+        return;
+    }
+    double bh[3], gradB[3], Jb[3], kappa[3], vperp[3], bk[3], bg[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bh[i] = B[i] / Bmag;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gradB[j] = (J[0][j] * bh[0] + J[1][j] * bh[1]) + J[2][j] * bh[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Jb[i] = (J[i][0] * bh[0] + J[i][1] * bh[1]) + J[i][2] * bh[2];
+    const double gb = dot3(gradB, bh);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) kappa[i] = (Jb[i] + bh[i] * (-gb)) / Bmag;
+    const double vpar = dot3(v, bh);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) vperp[i] = v[i] - vpar * bh[i];
+    const double wsq = dot3(vperp, vperp);
+    const double mu = (m * wsq) / (2.0 * Bmag);
+    cross3(bh, kappa, bk);
+    cross3(bh, gradB, bg);
+    out[0] = (q * vpar) * dot3(E, bh);
+    out[1] = ((m * (vpar * vpar)) / Bmag) * dot3(bk, E);
+    out[2] = (mu / Bmag) * dot3(bg, E);
+    out[3] = mu * dot3(dBdt, bh);
+}
+
+// ---------------------------------------------------------------------------
 // FP32 helpers (compensated path).  Contractions are written out with fmaf.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float dot3f(const float a[3], const float b[3])
@@ -369,10 +533,15 @@ __device__ __forceinline__ void boris_f32(const float v[3], const float E[3], co
     for (int c = 0; c < 3; ++c) out[c] = fmaf(h, E[c], vm[c] + vps[c]);
 }
 
+template <int NMAX>
 __device__ __forceinline__ float horner_f32(const double* p, int d, float u)
 {
-    float r = (float)p[d];
-    for (int j = d - 1; j >= 0; --j) r = fmaf(r, u, (float)p[j]);
+    float r = 0.0f;
+#pragma unroll
+    for (int j = NMAX; j >= 0; --j) {
+        if (j == d) r = (float)p[j];
+        else if (j < d) r = fmaf(r, u, (float)p[j]);
+    }
     return r;
 }
 
@@ -405,11 +574,11 @@ __device__ __forceinline__ void multistep_f32(const KParams& p, const float v[3]
     const float tau2 = dot3f(tn, tn);
     float c1, c2, c3, c6;
     if (p.form == 1) {
-        const float rD = 1.0f / horner_f32(p.poly.D, p.poly.deg, tau2);
-        c1 = horner_f32(p.poly.P1, p.poly.deg, tau2) * rD;
-        c2 = horner_f32(p.poly.P2, p.poly.deg - 1, tau2) * rD;
-        c3 = horner_f32(p.poly.P3, p.poly.deg - 1, tau2) * rD;
-        c6 = horner_f32(p.poly.P6, p.poly.deg - 1, tau2) * rD;
+        const float rD = 1.0f / horner_f32<kRationalNMax>(p.poly.D, p.poly.deg, tau2);
+        c1 = horner_f32<kRationalNMax>(p.poly.P1, p.poly.deg, tau2) * rD;
+        c2 = horner_f32<kRationalNMax - 1>(p.poly.P2, p.poly.deg - 1, tau2) * rD;
+        c3 = horner_f32<kRationalNMax - 1>(p.poly.P3, p.poly.deg - 1, tau2) * rD;
+        c6 = horner_f32<kRationalNMax - 1>(p.poly.P6, p.poly.deg - 1, tau2) * rD;
     } else {
         const float tmag = sqrtf(tau2);
         const float nn = (float)n;

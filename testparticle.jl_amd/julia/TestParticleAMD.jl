@@ -53,7 +53,10 @@ Base.@kwdef mutable struct SolveDesc
     nsteps_pp::Int64 = 0
     block_size::Int32 = 0
     flags::Int32 = 0
-    reserved::NTuple{6, Int32} = ntuple(_ -> Int32(0), 6)
+    mass::Float64 = 0.0
+    save_fields::Int32 = 0
+    save_work::Int32 = 0
+    reserved::NTuple{2, Int32} = ntuple(_ -> Int32(0), 2)
 end
 
 mutable struct Stats
@@ -151,7 +154,8 @@ end
 """
     solve(prob::TraceProblem, alg, backend::MI355X, [ensemblealg]; dt, trajectories=1,
           savestepinterval=1, isoutside=nothing, save_start=true, save_end=true,
-          save_everystep=true, maxiters=1_000_000, precision=:fp64)
+          save_everystep=true, save_fields=false, save_work=false, maxiters=1_000_000,
+          precision=:fp64)
 
 Device counterpart of `solve(prob, Boris(), backend; ...)`
 (src/boris/boris_kernel.jl:312-382) for Boris, MultistepBoris and the adaptive
@@ -160,9 +164,10 @@ variants.  `prob.p = (q2m, m, E::DeviceField, B::DeviceField, F)`.
 function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensemblealg = nothing;
         dt = 0.0, trajectories::Int = 1, savestepinterval::Int = 1, isoutside = nothing,
         save_start::Bool = true, save_end::Bool = true, save_everystep::Bool = true,
+        save_fields::Bool = false, save_work::Bool = false,
         maxiters::Int = 1_000_000, precision::Symbol = :fp64, nout_cap::Int = 0)
     ctx = context(backend.device)
-    q2m, _, E, B, _ = prob.p
+    q2m, m, E, B, _ = prob.p
     T = Float64
     # initial states: prob_func on the host, as the reference does (boris.jl:501-503)
     xv0 = Matrix{Float64}(undef, 6, trajectories)
@@ -172,7 +177,10 @@ function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensembleal
     d = SolveDesc(q2m = q2m, t0 = prob.tspan[1], t1 = prob.tspan[2], dt = dt,
                   savestepinterval = savestepinterval, maxiters = maxiters,
                   save_start = save_start, save_end = save_end, save_everystep = save_everystep,
-                  precision = precision === :fp32c ? 1 : 0)
+                  precision = precision === :fp32c ? 1 : 0, mass = m,
+                  save_fields = save_fields, save_work = save_work)
+    # record width, as _prepare_saved_data builds it (boris.jl:405-440)
+    W = 6 + (save_fields ? 6 : 0) + (save_work ? 4 : 0)
     if alg isa MultistepBoris
         d.alg = 1; d.order_N = typeof(alg).parameters[1]; d.nsub = alg.n
     end
@@ -190,7 +198,7 @@ function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensembleal
     cache = IdDict()
     hE, hB = handle(ctx, E, cache), handle(ctx, B, cache)
     cap = d.nout_cap
-    u = Array{Float64}(undef, 6, cap, trajectories)
+    u = Array{Float64}(undef, W, cap, trajectories)
     t = Array{Float64}(undef, cap, trajectories)
     len = Vector{Int32}(undef, trajectories)
     ret = Vector{Int32}(undef, trajectories)
@@ -205,7 +213,7 @@ function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensembleal
     sols = map(1:trajectories) do i
         n = len[i]
         ts = t[1:n, i]
-        us = [SVector{6, T}(u[:, k, i]) for k in 1:n]
+        us = [SVector{W, T}(u[:, k, i]) for k in 1:n]
         rcode = ret[i] == 0 ? ReturnCode.Success : ReturnCode.Terminated
         build_solution(prob, :boris, ts, us; interp = LinearInterpolation(ts, us), retcode = rcode)
     end

@@ -24,5 +24,6 @@ from .problem import (Boundary, EnsembleContext, InsideSphere, OutsideBox, Outsi
 from .solution import EnsembleSolution, TraceSolution
 from .solve import initial_states, make_desc, plan, solve, solve_batch
 from .adjoint import AdjointResult, loss_gradient, trajectory_loss
+from .postprocess import get_fields, get_work, record_eval
 
 __all__ = [n for n in dir() if not n.startswith("_")]

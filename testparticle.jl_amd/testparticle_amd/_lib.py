@@ -55,8 +55,13 @@ class SolveDesc(C.Structure):
         ("nout_cap", C.c_int64), ("r_in", C.c_double), ("r_out", C.c_double),
         ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
         ("dt_pp", C.c_void_p), ("nsteps_pp", C.c_int64), ("block_size", C.c_int32),
-        ("flags", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("flags", C.c_int32), ("mass", C.c_double), ("save_fields", C.c_int32),
+        ("save_work", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
+
+    @property
+    def record_width(self) -> int:
+        return 6 + (6 if self.save_fields else 0) + (4 if self.save_work else 0)
 
 
 class Stats(C.Structure):
@@ -71,7 +76,7 @@ EXPORTS = [
     "tpamd_abi_version", "tpamd_init", "tpamd_finalize", "tpamd_last_error",
     "tpamd_field_zero", "tpamd_field_uniform", "tpamd_field_affine", "tpamd_field_dipole",
     "tpamd_field_grid", "tpamd_field_grid_cells", "tpamd_field_uxb", "tpamd_field_eval",
-    "tpamd_field_free",
+    "tpamd_field_free", "tpamd_record_eval",
     "tpamd_plan", "tpamd_solve", "tpamd_solve_device", "tpamd_adjoint",
     "tpamd_batch_create", "tpamd_batch_autotune", "tpamd_batch_run", "tpamd_batch_free",
     "tpamd_solve_batch",
@@ -128,6 +133,8 @@ def load_library(path: str = LIB_PATH):
                                     d3, d3, C.POINTER(Stats)]
         L.tpamd_field_eval.argtypes = [vp, vp, vp, d3, i64, C.c_double, C.c_int32, d3]
         L.tpamd_field_free.argtypes = [vp, vp]
+        L.tpamd_record_eval.argtypes = [vp, vp, vp, C.c_double, C.c_double, d3, d3, i64, C.c_int32,
+                                        C.c_int32, d3]
         L.tpamd_plan.argtypes = [vp, C.POINTER(SolveDesc), C.POINTER(i64), C.POINTER(i64)]
         L.tpamd_solve.argtypes = [vp, C.POINTER(SolveDesc), vp, vp, vp, i64, vp, vp, vp, vp,
                                   C.POINTER(Stats)]

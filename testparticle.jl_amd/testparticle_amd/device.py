@@ -36,7 +36,8 @@ class DeviceEnsemble:
         self.desc.nout_cap = cap
         self.cap = cap
         self.xv0 = torch.from_numpy(np.ascontiguousarray(xv0, dtype=np.float64)).to(self.dev)
-        self.u = torch.empty((self.N, max(cap, 1), 6), dtype=torch.float64, device=self.dev)
+        self.u = torch.empty((self.N, max(cap, 1), self.desc.record_width), dtype=torch.float64,
+                             device=self.dev)
         self.t = torch.empty((self.N, max(cap, 1)), dtype=torch.float64, device=self.dev)
         self.len = torch.empty(self.N, dtype=torch.int32, device=self.dev)
         self.ret = torch.empty(self.N, dtype=torch.int32, device=self.dev)
@@ -94,7 +95,8 @@ class DeviceBatch:
         self.cap = int(cap)
         self.desc.nout_cap = self.cap
         self.xv0 = torch.from_numpy(np.ascontiguousarray(np.concatenate(states))).to(self.dev)
-        self.u = torch.empty((self.N, max(self.cap, 1), 6), dtype=torch.float64, device=self.dev)
+        self.u = torch.empty((self.N, max(self.cap, 1), self.desc.record_width), dtype=torch.float64,
+                             device=self.dev)
         self.t = torch.empty((self.N, max(self.cap, 1)), dtype=torch.float64, device=self.dev)
         self.len = torch.empty(self.N, dtype=torch.int32, device=self.dev)
         self.ret = torch.empty(self.N, dtype=torch.int32, device=self.dev)

@@ -9,7 +9,8 @@ from ._lib import RETCODE_NAMES
 
 
 class TraceSolution:
-    """One trajectory: `t` (n,), `u` (n, 6) rows (x, y, z, vx, vy, vz),
+    """One trajectory: `t` (n,), `u` (n, w) rows (x, y, z, vx, vy, vz, then E and B
+    with save_fields, then P_par, P_fermi, P_grad, P_betatron with save_work),
     `retcode` ("Success" / "Terminated" / "MaxIters").  Calling it
     interpolates linearly between saved records, as the reference's
     LinearInterpolation(tsave, traj) does."""
@@ -53,7 +54,7 @@ class TraceSolution:
 class EnsembleSolution:
     """EnsembleSolution(sols, elapsed_time, converged): `u` is the list of
     TraceSolution (built on first access); `t_all` (N, cap), `u_all`
-    (N, cap, 6), `lengths` (N,) and `retcodes` (N,) expose the raw arrays."""
+    (N, cap, w), `lengths` (N,) and `retcodes` (N,) expose the raw arrays."""
 
     def __init__(self, t_all, u_all, lengths, retcodes, elapsed_time, stats=None, prob=None,
                  alg=None):

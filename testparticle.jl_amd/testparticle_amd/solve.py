@@ -67,9 +67,13 @@ def make_desc(prob: TraceProblem, alg: AbstractBoris, *, dt=None, savestepinterv
               isoutside=None, save_start=True, save_end=True, save_everystep=True,
               maxiters=1_000_000, precision="fp64", form="rational", max_steps=0,
               nout_cap=0, dts=None, nsteps=0, workgroup_size=0,
-              kernels="auto") -> _lib.SolveDesc:
+              kernels="auto", save_fields=False, save_work=False) -> _lib.SolveDesc:
     q2m = float(prob.p[0])
     d = _lib.SolveDesc()
+    # records gain E, B and/or the work rates (src/boris/boris.jl:405-440);
+    # the work rates need the mass m = p[2]
+    d.save_fields, d.save_work = int(bool(save_fields)), int(bool(save_work))
+    d.mass = float(prob.p[1]) if len(prob.p) > 1 else 0.0
     if isinstance(alg, MultistepBoris):
         d.alg, d.order_N, d.nsub = _lib.ALG_MULTISTEP, alg.N, alg.n
     elif isinstance(alg, Boris):
@@ -161,8 +165,6 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
     fixed-step only), `nout_cap` (adaptive record capacity), `kernels`
     ("auto" | "generic": skip the gridded-field specialisations)."""
     backend, _ens = _split_positional(args)
-    if save_fields or save_work:
-        raise NotImplementedError("save_fields / save_work are not provided by the device path")
     if alg.adaptive and dt is not None:
         raise TypeError("the adaptive solver takes no dt (src/boris/adaptive_boris.jl:21-33)")
     dev = backend.device if backend is not None else (0 if device is None else int(device))
@@ -176,7 +178,7 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
                      save_start=save_start, save_end=save_end, save_everystep=save_everystep,
                      maxiters=maxiters, precision=precision, form=form, max_steps=max_steps,
                      dts=dts_arr, nsteps=nsteps, workgroup_size=workgroup_size,
-                     kernels=kernels)
+                     kernels=kernels, save_fields=save_fields, save_work=save_work)
     if dts_arr is not None:
         desc.dt_pp = dts_arr.ctypes.data
     if not alg.adaptive:
@@ -194,7 +196,7 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
     hE, hB = E.handle(ctx), B.handle(ctx)
     while True:
         desc.nout_cap = cap
-        u = np.empty((N, max(cap, 0), 6))
+        u = np.empty((N, max(cap, 0), desc.record_width))
         t = np.empty((N, max(cap, 0)))
         ln = np.zeros(N, dtype=np.int32)
         rc = np.zeros(N, dtype=np.int32)
@@ -223,7 +225,8 @@ def solve_batch(problems, alg: AbstractBoris, *, dt=None, savestepinterval: int 
                 isoutside=None, save_start: bool = True, save_end: bool = True,
                 save_everystep: bool = True, maxiters: int = 1_000_000,
                 precision: str = "fp64", form: str = "rational", max_steps: int = 0,
-                nout_cap=None, kernels: str = "auto", device: int = 0):
+                nout_cap=None, kernels: str = "auto", device: int = 0,
+                save_fields: bool = False, save_work: bool = False):
     """Solve many problems, each with its own fields and (N_k, 6) initial
     states, in one persistent launch (tpamd_solve_batch).  All problems share
     q/m, tspan and the solver keywords.  Returns one EnsembleSolution per
@@ -247,7 +250,8 @@ def solve_batch(problems, alg: AbstractBoris, *, dt=None, savestepinterval: int 
     desc = make_desc(problems[0], alg, dt=dt, savestepinterval=savestepinterval,
                      isoutside=isoutside, save_start=save_start, save_end=save_end,
                      save_everystep=save_everystep, maxiters=maxiters, precision=precision,
-                     form=form, max_steps=max_steps, kernels=kernels)
+                     form=form, max_steps=max_steps, kernels=kernels, save_fields=save_fields,
+                     save_work=save_work)
     if not alg.adaptive:
         cap = plan(desc)[1]
     elif nout_cap is not None:
@@ -259,7 +263,7 @@ def solve_batch(problems, alg: AbstractBoris, *, dt=None, savestepinterval: int 
     hB = (C.c_void_p * len(problems))(*[f[1].handle(ctx).value for f in fields])
     while True:
         desc.nout_cap = cap
-        u = np.empty((N, max(cap, 0), 6))
+        u = np.empty((N, max(cap, 0), desc.record_width))
         t = np.empty((N, max(cap, 0)))
         ln = np.zeros(N, dtype=np.int32)
         rc = np.zeros(N, dtype=np.int32)

@@ -34,7 +34,8 @@ def oracle_field(f, B_oracle=None):
 
 def run_oracle(prob, alg, *, dt=None, savestepinterval=1, isoutside=None, save_start=True,
                save_end=True, save_everystep=True, maxiters=1_000_000, form="rational",
-               max_steps=0, nout_cap=None, dts=None, nsteps=0, nthreads=8, trajectories=1):
+               max_steps=0, nout_cap=None, dts=None, nsteps=0, nthreads=8, trajectories=1,
+               save_fields=False, save_work=False):
     xv0 = TP.initial_states(prob, trajectories)     # host-side prob_func, no device
     B = oracle_field(prob.p[3])
     E = oracle_field(prob.p[2], B)
@@ -63,7 +64,8 @@ def run_oracle(prob, alg, *, dt=None, savestepinterval=1, isoutside=None, save_s
     return O.solve(xv0, E, B, q2m=prob.p[0], tspan=prob.tspan, dt=dt or 0.0,
                    savestepinterval=savestepinterval, save_start=save_start, save_end=save_end,
                    save_everystep=save_everystep, maxiters=maxiters, dt_pp=dts,
-                   nsteps_pp=nsteps, nthreads=nthreads, **kw)
+                   nsteps_pp=nsteps, nthreads=nthreads, save_fields=save_fields,
+                   save_work=save_work, mass=float(prob.p[1]), **kw)
 
 
 def assert_bitwise(gpu: TP.EnsembleSolution, orc: O.OracleResult, n_check=None):
@@ -71,6 +73,7 @@ def assert_bitwise(gpu: TP.EnsembleSolution, orc: O.OracleResult, n_check=None):
     assert orc.len.shape[0] == N
     np.testing.assert_array_equal(gpu.lengths, orc.len)
     np.testing.assert_array_equal(gpu.retcodes, orc.retcode)
+    assert gpu.u_all.shape[2] == orc.u.shape[2], "record widths differ"
     cap = min(gpu.u_all.shape[1], orc.u.shape[1])
     idx = range(N) if n_check is None else np.linspace(0, N - 1, n_check).astype(int)
     for i in idx:

@@ -164,10 +164,20 @@ def test_solve_rejects_host_callables():
     prob = TP.TraceProblem([0] * 6, (0, 1), (1.0, 1.0, TP.ZeroField(), TP.ZeroField(), None))
     with pytest.raises(TypeError):
         TP.solve(prob, TP.Boris(), dt=0.1, isoutside=lambda u, p, t: False)
-    with pytest.raises(NotImplementedError):
-        TP.solve(prob, TP.Boris(), dt=0.1, save_fields=True)
     with pytest.raises(TypeError):
         TP.solve(prob, TP.AdaptiveBoris(), dt=0.1)
+
+
+def test_record_width_and_mass_in_descriptor():
+    # _prepare_saved_data (src/boris/boris.jl:405-440): 6 + 6 (E, B) + 4 (work)
+    z = TP.ZeroField()
+    prob = TP.TraceProblem([0.0] * 6, (0.0, 1.0), (2.5, 7.0, z, z, z))
+    for sf, sw, w in ((False, False, 6), (True, False, 12), (False, True, 10), (True, True, 16)):
+        d = TP.make_desc(prob, TP.Boris(), dt=0.1, save_fields=sf, save_work=sw)
+        assert d.record_width == w
+        assert d.mass == 7.0 and d.q2m == 2.5
+    # the C struct keeps its size: the new fields took the reserved words
+    assert C.sizeof(TP._lib.SolveDesc) == 224
 
 
 def test_solve_batch_argument_checks():
