@@ -52,7 +52,8 @@ typedef enum tpamd_field_kind {
     TPAMD_FIELD_AFFINE = 2,   /* F_i = f0_i + sum_j J_ij x_j + ft_i t */
     TPAMD_FIELD_DIPOLE = 3,   /* B = b0 R^3/r^5 (-3xz, -3yz, r^2-3z^2) */
     TPAMD_FIELD_GRID = 4,     /* FieldInterpolator over a uniform CartesianGrid, order 1 */
-    TPAMD_FIELD_UXB = 5       /* E = -u x B(x), B the solve's magnetic field */
+    TPAMD_FIELD_UXB = 5,      /* E = -u x B(x), B the solve's magnetic field */
+    TPAMD_FIELD_TIME_GRID = 6 /* LazyTimeInterpolator over gridded snapshots */
 } tpamd_field_kind;
 
 /* Extrapolation of gridded fields, src/utility/interpolation.jl:203-206 */
@@ -175,6 +176,23 @@ int32_t tpamd_field_grid(tpamd_ctx* ctx, const void* data, int32_t dtype, int64_
                          int64_t ny, int64_t nz, const double origin[3],
                          const double spacing[3], int32_t extrap, int32_t storage_dtype,
                          tpamd_field** out);
+/* Time-dependent gridded field: the device counterpart of
+ * LazyTimeInterpolator(times, loader) whose loader(k) returns
+ * build_interpolator(CartesianGrid, A_k, x, y, z, 1, bc) on one grid
+ * (src/utility/interpolation.jl:337-430, test/test_utility.jl:312-404).
+ * snapshots[k] is the host array of snapshot k (0-based), laid out as for
+ * tpamd_field_grid; times[nsnap] are finite and strictly increasing.  All
+ * snapshots are copied to the device and stay resident there (the reference
+ * loads them lazily and keeps at most three in memory).  At time t, with k = searchsortedlast(times, t):
+ * k == 0 gives snapshot 1, k == nsnap snapshot nsnap, otherwise
+ * (1 - w) A_k(x) + w A_{k+1}(x) with w = (t - t_k) / (t_{k+1} - t_k).  Its
+ * Jacobian blends the same way and its time derivative is
+ * (A_{k+1}(x) - A_k(x)) / (t_{k+1} - t_k) inside the time span, 0 outside. */
+int32_t tpamd_field_time_grid(tpamd_ctx* ctx, const void* const* snapshots, int32_t dtype,
+                              int64_t nx, int64_t ny, int64_t nz, int32_t nsnap,
+                              const double* times, const double origin[3],
+                              const double spacing[3], int32_t extrap, int32_t storage_dtype,
+                              tpamd_field** out);
 /* Builds (enable = 1) or drops (0) a cell-replicated copy of a gridded field:
  * per cell the 8 corner nodes x 3 components in one block (float: 128 B, one
  * cache line; double: 192 B), so a trilinear gather reads one contiguous

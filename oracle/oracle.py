@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libtporacle.so")
 
 FIELD_ZERO, FIELD_UNIFORM, FIELD_AFFINE, FIELD_DIPOLE, FIELD_GRID, FIELD_UXB = 0, 1, 2, 3, 4, 5
+FIELD_TIME_GRID = 6
 FIELD_KAT_FRONT, FIELD_KAT_SWITCH_ON, FIELD_KAT_NAN_OUTSIDE = 100, 101, 102
 EXTRAP_FILL_NAN, EXTRAP_CLAMP, EXTRAP_PERIODIC = 0, 1, 2
 ALG_BORIS, ALG_MULTISTEP = 0, 1
@@ -38,6 +39,7 @@ OrcField._fields_ = [
     ("data", C.c_void_p), ("nx", C.c_long), ("ny", C.c_long), ("nz", C.c_long),
     ("origin", C.c_double * 3), ("spacing", C.c_double * 3),
     ("src", C.POINTER(OrcField)),
+    ("times", C.POINTER(C.c_double)), ("nsnap", C.c_long),
 ]
 
 
@@ -139,6 +141,23 @@ def grid(data: np.ndarray, origin, spacing, extrap: int) -> Field:
     st.origin[:] = [float(a) for a in origin]
     st.spacing[:] = [float(a) for a in spacing]
     return Field(st, keep=(data,))
+
+
+def time_grid(snaps: np.ndarray, times, origin, spacing, extrap: int) -> Field:
+    """LazyTimeInterpolator over gridded snapshots
+    (src/utility/interpolation.jl:337-430).  snaps: (nsnap, nz, ny, nx, 3)
+    (C order), snapshot k the grid at times[k]."""
+    assert snaps.ndim == 5 and snaps.shape[4] == 3
+    assert snaps.dtype in (np.float64, np.float32)
+    snaps = np.ascontiguousarray(snaps)
+    tm = np.ascontiguousarray(times, dtype=np.float64)
+    assert tm.shape == (snaps.shape[0],) and np.all(np.diff(tm) > 0)
+    _, nz, ny, nx, _ = snaps.shape
+    st = OrcField(kind=FIELD_TIME_GRID, extrap=extrap, dtype=0 if snaps.dtype == np.float64 else 1,
+                  data=snaps.ctypes.data, nx=nx, ny=ny, nz=nz, times=_dp(tm), nsnap=len(tm))
+    st.origin[:] = [float(a) for a in origin]
+    st.spacing[:] = [float(a) for a in spacing]
+    return Field(st, keep=(snaps, tm))
 
 
 def uxb(u, B: Field) -> Field:

@@ -271,6 +271,44 @@ static void grid_eval(const orc_field* f, const double x[3], double out[3])
     }
 }
 
+/* ----------------------------------------------------------------------------
+ * LazyTimeInterpolator, src/utility/interpolation.jl:337-430: snapshot k
+ * (0-based) is the grid of the same geometry at data + k * 3 nx ny nz values.
+ * time_index is searchsortedlast(times, t) (interpolation.jl:360): the number
+ * of snapshot times <= t, found by a linear scan.
+ * ------------------------------------------------------------------------- */
+static orc_field snapshot_of(const orc_field* f, long k)
+{
+    orc_field g = *f;
+    const size_t es = f->dtype == ORC_DTYPE_F32 ? sizeof(float) : sizeof(double);
+    g.kind = ORC_FIELD_GRID;
+    g.data = (const char*)f->data + (size_t)k * (size_t)(3 * f->nx * f->ny * f->nz) * es;
+    return g;
+}
+
+static long time_index(const orc_field* f, double t)
+{
+    long k = 0;
+    while (k < f->nsnap && f->times[k] <= t) ++k;
+    return k;
+}
+
+static void time_grid_eval(const orc_field* f, const double x[3], double t, double out[3])
+{
+    const long k = time_index(f, t);
+    if (k == 0 || k >= f->nsnap) {                       /* :363-367 clamp */
+        const orc_field g = snapshot_of(f, k == 0 ? 0 : f->nsnap - 1);
+        grid_eval(&g, x, out);
+        return;
+    }
+    const double w = (t - f->times[k - 1]) / (f->times[k] - f->times[k - 1]);   /* :372 */
+    const orc_field g1 = snapshot_of(f, k - 1), g2 = snapshot_of(f, k);
+    double a[3], b[3];
+    grid_eval(&g1, x, a);
+    grid_eval(&g2, x, b);
+    for (int c = 0; c < 3; ++c) out[c] = (1.0 - w) * a[c] + w * b[c];     /* :378 */
+}
+
 void orc_field_eval(const orc_field* f, const double x[3], double t, double out[3])
 {
     switch (f->kind) {
@@ -301,6 +339,9 @@ void orc_field_eval(const orc_field* f, const double x[3], double t, double out[
     }
     case ORC_FIELD_GRID:
         grid_eval(f, x, out);
+        return;
+    case ORC_FIELD_TIME_GRID:
+        time_grid_eval(f, x, t, out);
         return;
     case ORC_FIELD_UXB: {                      /* E = -u x B = B x u */
         double B[3];
@@ -362,6 +403,27 @@ void orc_field_jac(const orc_field* f, const double x[3], double t, double B[3],
                               2.0 * x[0], 2.0 * x[1], -4.0 * x[2]};
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) J[3 * i + j] = a * dg[3 * i + j] - (b * x[j]) * g[i];
+        return;
+    }
+    case ORC_FIELD_TIME_GRID: {
+        /* jacobian / derivative_t of LazyTimeInterpolator
+         * (interpolation.jl:396-430): clamped outside the time span with a
+         * zero time derivative, else the w-blend of the two snapshots'
+         * Jacobians and their difference quotient in time */
+        double Bs[3], J2[9], d2[3], a[3], b[3];
+        const long k = time_index(f, t);
+        if (k == 0 || k >= f->nsnap) {
+            const orc_field g = snapshot_of(f, k == 0 ? 0 : f->nsnap - 1);
+            orc_field_jac(&g, x, t, Bs, J, d2);
+            return;
+        }
+        const double w = (t - f->times[k - 1]) / (f->times[k] - f->times[k - 1]);
+        const orc_field g1 = snapshot_of(f, k - 1), g2 = snapshot_of(f, k);
+        orc_field_jac(&g1, x, t, a, J, d2);
+        orc_field_jac(&g2, x, t, b, J2, d2);
+        for (int i = 0; i < 9; ++i) J[i] = (1.0 - w) * J[i] + w * J2[i];
+        const double dtk = f->times[k] - f->times[k - 1];
+        for (int i = 0; i < 3; ++i) dBdt[i] = (b[i] - a[i]) / dtk;
         return;
     }
     case ORC_FIELD_GRID: {

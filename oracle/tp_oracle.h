@@ -26,6 +26,7 @@ enum {
     ORC_FIELD_DIPOLE = 3,    /* B = b0 R^3 / r^5 (-3xz, -3yz, r^2 - 3z^2) */
     ORC_FIELD_GRID = 4,      /* trilinear on a uniform Cartesian grid */
     ORC_FIELD_UXB = 5,       /* E = -u x B(x, t), B = *src */
+    ORC_FIELD_TIME_GRID = 6, /* LazyTimeInterpolator over gridded snapshots */
     ORC_FIELD_KAT_FRONT = 100,      /* test_boris.jl:12-21 time_varying_B */
     ORC_FIELD_KAT_SWITCH_ON = 101,  /* test_boris.jl:252 B = t > t_on ? v : 0 */
     ORC_FIELD_KAT_NAN_OUTSIDE = 102 /* test_boundary.jl:10-26 */
@@ -50,6 +51,10 @@ typedef struct orc_field {
     double origin[3];
     double spacing[3];
     const struct orc_field* src;
+    /* TIME_GRID: nsnap grids of the geometry above stored back to back in
+     * data (3 nx ny nz values each), at strictly increasing times[nsnap] */
+    const double* times;
+    long nsnap;
 } orc_field;
 
 enum { ORC_ALG_BORIS = 0, ORC_ALG_MULTISTEP = 1 };

@@ -1062,6 +1062,8 @@ int check_desc(tpamd_ctx* ctx, const tpamd_solve_desc* d, int64_t* nt, int64_t* 
     return TPAMD_OK;
 }
 
+bool is_gridded(const DevField& f) { return f.kind == kFieldGrid || f.kind == kFieldTimeGrid; }
+
 bool same_geometry(const DevField& a, const DevField& b)
 {
     for (int d = 0; d < 3; ++d)
@@ -1082,8 +1084,10 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
             return fail(ctx, TPAMD_ERR_UNSUPPORTED, "UxB electric field must derive from the solve's B");
         p.e_is_uxb = 1;
     }
-    // only the affine field can depend on t (its ft term); a UxB E follows B
+    // an affine field with a non-zero ft term and a snapshot sequence depend
+    // on t; a UxB E follows B
     auto is_static = [](const DevField& f) {
+        if (f.kind == kFieldTimeGrid) return f.nsnap == 1;
         return f.kind != kFieldAffine || (f.ft[0] == 0.0 && f.ft[1] == 0.0 && f.ft[2] == 0.0);
     };
     p.static_fields = is_static(B->d) && (p.e_is_uxb || is_static(E->d)) ? 1 : 0;
@@ -1091,8 +1095,8 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
     p.frame_periodic = 0;
     if (precision == TPAMD_FP32C) {
         const DevField* g = nullptr;
-        if (B->d.kind == kFieldGrid) g = &B->d;
-        if (!p.e_is_uxb && E->d.kind == kFieldGrid) {
+        if (is_gridded(B->d)) g = &B->d;
+        if (!p.e_is_uxb && is_gridded(E->d)) {
             if (g && !same_geometry(*g, E->d))
                 return fail(ctx, TPAMD_ERR_UNSUPPORTED,
                             "FP32C needs gridded E and B on the same grid");
@@ -1350,7 +1354,7 @@ int fill_params(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field*
     for (int d = 0; d < 3; ++d) { p.box_lo[d] = desc->box_lo[d]; p.box_hi[d] = desc->box_hi[d]; }
     if (desc->bnd_flags & TPAMD_BND_GRID) {
         // open boundary at the faces of B's grid: the node span becomes the box
-        if (B->d.kind != kFieldGrid)
+        if (!is_gridded(B->d))
             return fail(ctx, TPAMD_ERR_ARGUMENT, "TPAMD_BND_GRID needs a gridded B field");
         for (int d = 0; d < 3; ++d) {
             const double lo = B->d.origin[d];
@@ -1757,12 +1761,12 @@ int32_t tpamd_field_dipole(tpamd_ctx* ctx, double b0, double radius, tpamd_field
     return TPAMD_OK;
 }
 
-int32_t tpamd_field_grid(tpamd_ctx* ctx, const void* data, int32_t dtype, int64_t nx, int64_t ny,
-                         int64_t nz, const double origin[3], const double spacing[3],
-                         int32_t extrap, int32_t storage_dtype, tpamd_field** out)
+namespace {
+
+int check_grid_args(tpamd_ctx* ctx, int32_t dtype, int64_t nx, int64_t ny, int64_t nz,
+                    const double origin[3], const double spacing[3], int32_t extrap,
+                    int32_t storage_dtype)
 {
-    if (!ctx || !out || !data || !origin || !spacing)
-        return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
     if (nx < 2 || ny < 2 || nz < 2)
         return fail(ctx, TPAMD_ERR_ARGUMENT, "grid needs at least 2 nodes per dimension");
     if (nx > (1 << 30) || ny > (1 << 30) || nz > (1 << 30))
@@ -1774,41 +1778,35 @@ int32_t tpamd_field_grid(tpamd_ctx* ctx, const void* data, int32_t dtype, int64_
     for (int d = 0; d < 3; ++d)
         if (!(spacing[d] > 0) || !std::isfinite(origin[d]))
             return fail(ctx, TPAMD_ERR_ARGUMENT, "grid spacing must be > 0 and origin finite");
-    const size_t nn = (size_t)nx * (size_t)ny * (size_t)nz * 3;
+    return TPAMD_OK;
+}
+
+// copies nn host values of `dtype` to dev as `storage_dtype` (converted on the
+// host in chunks when the types differ: a one-time set-up cost)
+hipError_t upload_nodes(void* dev, const void* data, int32_t dtype, size_t nn, int32_t storage_dtype)
+{
     const size_t es = storage_dtype == TPAMD_F64 ? 8 : 4;
-    void* dev = nullptr;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipMalloc(&dev, nn * es));
-    hipError_t e;
-    if (dtype == storage_dtype) {
-        e = hipMemcpy(dev, data, nn * es, hipMemcpyHostToDevice);
-    } else {
-        // convert on the host in chunks (one-time set-up cost)
-        const size_t chunk = 1 << 22;
-        char* tmp = (char*)std::malloc(chunk * es);
-        e = tmp ? hipSuccess : hipErrorOutOfMemory;
-        for (size_t off = 0; e == hipSuccess && off < nn; off += chunk) {
-            const size_t m = (nn - off) < chunk ? (nn - off) : chunk;
-            for (size_t j = 0; j < m; ++j) {
-                const double val = dtype == TPAMD_F64 ? ((const double*)data)[off + j]
-                                                      : (double)((const float*)data)[off + j];
-                if (es == 8) ((double*)tmp)[j] = val;
-                else ((float*)tmp)[j] = (float)val;
-            }
-            e = hipMemcpy((char*)dev + off * es, tmp, m * es, hipMemcpyHostToDevice);
+    if (dtype == storage_dtype) return hipMemcpy(dev, data, nn * es, hipMemcpyHostToDevice);
+    const size_t chunk = 1 << 22;
+    char* tmp = (char*)std::malloc(chunk * es);
+    hipError_t e = tmp ? hipSuccess : hipErrorOutOfMemory;
+    for (size_t off = 0; e == hipSuccess && off < nn; off += chunk) {
+        const size_t m = (nn - off) < chunk ? (nn - off) : chunk;
+        for (size_t j = 0; j < m; ++j) {
+            const double val = dtype == TPAMD_F64 ? ((const double*)data)[off + j]
+                                                  : (double)((const float*)data)[off + j];
+            if (es == 8) ((double*)tmp)[j] = val;
+            else ((float*)tmp)[j] = (float)val;
         }
-        std::free(tmp);
+        e = hipMemcpy((char*)dev + off * es, tmp, m * es, hipMemcpyHostToDevice);
     }
-    if (e != hipSuccess) {
-        (void)hipFree(dev);
-        return fail(ctx, TPAMD_ERR_DEVICE, "grid upload failed: %s", hipGetErrorString(e));
-    }
-    int rc = new_field(ctx, out, kFieldGrid);
-    if (rc) {
-        (void)hipFree(dev);
-        return rc;
-    }
-    DevField& f = (*out)->d;
+    std::free(tmp);
+    return e;
+}
+
+void set_grid_geometry(DevField& f, int64_t nx, int64_t ny, int64_t nz, const double origin[3],
+                       const double spacing[3], int32_t extrap, int32_t storage_dtype)
+{
     f.extrap = extrap;
     f.dtype = storage_dtype == TPAMD_F64 ? 0 : 1;
     f.n[0] = (int)nx; f.n[1] = (int)ny; f.n[2] = (int)nz;
@@ -1818,9 +1816,87 @@ int32_t tpamd_field_grid(tpamd_ctx* ctx, const void* data, int32_t dtype, int64_
         f.inv_hf[d] = (float)f.inv_h[d];
         f.period[d] = (double)f.n[d] * spacing[d];
     }
-    f.data = dev;
+}
+
+} // namespace
+
+int32_t tpamd_field_grid(tpamd_ctx* ctx, const void* data, int32_t dtype, int64_t nx, int64_t ny,
+                         int64_t nz, const double origin[3], const double spacing[3],
+                         int32_t extrap, int32_t storage_dtype, tpamd_field** out)
+{
+    if (!ctx || !out || !data || !origin || !spacing)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
+    int rc = check_grid_args(ctx, dtype, nx, ny, nz, origin, spacing, extrap, storage_dtype);
+    if (rc) return rc;
+    const size_t nn = (size_t)nx * (size_t)ny * (size_t)nz * 3;
+    const size_t es = storage_dtype == TPAMD_F64 ? 8 : 4;
+    void* dev = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMalloc(&dev, nn * es));
+    const hipError_t e = upload_nodes(dev, data, dtype, nn, storage_dtype);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return fail(ctx, TPAMD_ERR_DEVICE, "grid upload failed: %s", hipGetErrorString(e));
+    }
+    rc = new_field(ctx, out, kFieldGrid);
+    if (rc) {
+        (void)hipFree(dev);
+        return rc;
+    }
+    set_grid_geometry((*out)->d, nx, ny, nz, origin, spacing, extrap, storage_dtype);
+    (*out)->d.data = dev;
     (*out)->dev_data = dev;
     (*out)->bytes = nn * es;
+    return TPAMD_OK;
+}
+
+int32_t tpamd_field_time_grid(tpamd_ctx* ctx, const void* const* snapshots, int32_t dtype,
+                              int64_t nx, int64_t ny, int64_t nz, int32_t nsnap,
+                              const double* times, const double origin[3],
+                              const double spacing[3], int32_t extrap, int32_t storage_dtype,
+                              tpamd_field** out)
+{
+    if (!ctx || !out || !snapshots || !times || !origin || !spacing)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
+    if (nsnap < 1) return fail(ctx, TPAMD_ERR_ARGUMENT, "a time-dependent grid needs >= 1 snapshot");
+    for (int k = 0; k < nsnap; ++k) {
+        if (!snapshots[k]) return fail(ctx, TPAMD_ERR_ARGUMENT, "null snapshot %d", k);
+        if (!std::isfinite(times[k]))
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "snapshot times must be finite");
+        if (k > 0 && !(times[k] > times[k - 1]))
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "snapshot times must be strictly increasing");
+    }
+    int rc = check_grid_args(ctx, dtype, nx, ny, nz, origin, spacing, extrap, storage_dtype);
+    if (rc) return rc;
+    const size_t nn = (size_t)nx * (size_t)ny * (size_t)nz * 3;
+    const size_t es = storage_dtype == TPAMD_F64 ? 8 : 4;
+    // one allocation: the times, padded to 256 B, then the snapshots, each
+    // padded to 256 B so every snapshot starts on its own cache-line boundary
+    const size_t head = ((size_t)nsnap * 8 + 255) / 256 * 256;
+    const size_t snap = (nn * es + 255) / 256 * 256;
+    void* dev = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMalloc(&dev, head + snap * (size_t)nsnap));
+    hipError_t e = hipMemcpy(dev, times, (size_t)nsnap * 8, hipMemcpyHostToDevice);
+    for (int k = 0; e == hipSuccess && k < nsnap; ++k)
+        e = upload_nodes((char*)dev + head + (size_t)k * snap, snapshots[k], dtype, nn, storage_dtype);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return fail(ctx, TPAMD_ERR_DEVICE, "grid upload failed: %s", hipGetErrorString(e));
+    }
+    rc = new_field(ctx, out, kFieldTimeGrid);
+    if (rc) {
+        (void)hipFree(dev);
+        return rc;
+    }
+    DevField& f = (*out)->d;
+    set_grid_geometry(f, nx, ny, nz, origin, spacing, extrap, storage_dtype);
+    f.times = (const double*)dev;
+    f.data = (const char*)dev + head;
+    f.snap_bytes = (long long)snap;
+    f.nsnap = nsnap;
+    (*out)->dev_data = dev;
+    (*out)->bytes = head + snap * (size_t)nsnap;
     return TPAMD_OK;
 }
 
@@ -1848,7 +1924,8 @@ int32_t tpamd_field_free(tpamd_ctx* ctx, tpamd_field* f)
 int32_t tpamd_field_grid_cells(tpamd_ctx* ctx, tpamd_field* f, int32_t enable)
 {
     if (!ctx || !f) return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
-    if (f->d.kind != kFieldGrid) return fail(ctx, TPAMD_ERR_ARGUMENT, "not a gridded field");
+    if (f->d.kind != kFieldGrid)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "the cell layout needs a static gridded field");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (!enable) {
         if (f->cells_data) HIP_TRY(ctx, hipFree(f->cells_data));

@@ -50,6 +50,12 @@ struct DevField {
     const void* cells;
     int nc[3];
     unsigned cell_stride;
+    // time-dependent grids (kind TimeGrid): nsnap snapshots of the node
+    // array, snap_bytes apart from `data`, at sorted times[nsnap]
+    const double* times;
+    long long snap_bytes;
+    int nsnap;
+    int pad3;
 };
 
 struct Poly {
@@ -105,7 +111,7 @@ struct KParams {
 };
 
 enum { kFieldZero = 0, kFieldUniform = 1, kFieldAffine = 2, kFieldDipole = 3, kFieldGrid = 4,
-       kFieldUxB = 5 };
+       kFieldUxB = 5, kFieldTimeGrid = 6 };
 enum { kExtrapFillNaN = 0, kExtrapClamp = 1, kExtrapPeriodic = 2 };
 enum { kBndNaN = 1, kBndSphereIn = 2, kBndSphereOut = 4, kBndBox = 8 };
 enum { kRetSuccess = 0, kRetTerminated = 1, kRetMaxIters = 2 };
@@ -240,21 +246,28 @@ __device__ __forceinline__ double lerp_f64(double a, double b, double w)
     return fma(w, b, (1.0 - w) * a);
 }
 
-__device__ __forceinline__ void node3_f64(const DevField& f, long long idx, double o[3])
+__device__ __forceinline__ void node3_f64(const DevField& f, const void* data, long long idx,
+                                          double o[3])
 {
     if (f.dtype == 1) {
-        const float* d = (const float*)f.data + 3 * idx;
+        const float* d = (const float*)data + 3 * idx;
         o[0] = (double)d[0]; o[1] = (double)d[1]; o[2] = (double)d[2];
     } else {
-        const double* d = (const double*)f.data + 3 * idx;
+        const double* d = (const double*)data + 3 * idx;
         o[0] = d[0]; o[1] = d[1]; o[2] = d[2];
     }
+}
+__device__ __forceinline__ void node3_f64(const DevField& f, long long idx, double o[3])
+{
+    node3_f64(f, f.data, idx, o);
 }
 
 // build_interpolator(CartesianGrid, A::4D, ...) + linear_interp,
 // src/utility/interpolation.jl:119-148, :203-206, :217-235
-__device__ __forceinline__ void grid_f64(const DevField& f, const double x[3], double out[3])
+__device__ __forceinline__ void grid_f64(const DevField& f, const double x[3], double out[3],
+                                         const void* data = nullptr)
 {
+    const void* base = data ? data : f.data;
     int i0[3], i1[3];
     double w[3];
 #pragma unroll
@@ -292,25 +305,46 @@ __device__ __forceinline__ void grid_f64(const DevField& f, const double x[3], d
     }
     const long long nx = f.n[0], nxy = (long long)f.n[0] * f.n[1];
     double a[3], b[3], c00[3], c10[3], c01[3], c11[3];
-    node3_f64(f, i0[0] + nx * i0[1] + nxy * i0[2], a);
-    node3_f64(f, i1[0] + nx * i0[1] + nxy * i0[2], b);
+    node3_f64(f, base, i0[0] + nx * i0[1] + nxy * i0[2], a);
+    node3_f64(f, base, i1[0] + nx * i0[1] + nxy * i0[2], b);
 #pragma unroll
     for (int c = 0; c < 3; ++c) c00[c] = lerp_f64(a[c], b[c], w[0]);
-    node3_f64(f, i0[0] + nx * i1[1] + nxy * i0[2], a);
-    node3_f64(f, i1[0] + nx * i1[1] + nxy * i0[2], b);
+    node3_f64(f, base, i0[0] + nx * i1[1] + nxy * i0[2], a);
+    node3_f64(f, base, i1[0] + nx * i1[1] + nxy * i0[2], b);
 #pragma unroll
     for (int c = 0; c < 3; ++c) c10[c] = lerp_f64(a[c], b[c], w[0]);
-    node3_f64(f, i0[0] + nx * i0[1] + nxy * i1[2], a);
-    node3_f64(f, i1[0] + nx * i0[1] + nxy * i1[2], b);
+    node3_f64(f, base, i0[0] + nx * i0[1] + nxy * i1[2], a);
+    node3_f64(f, base, i1[0] + nx * i0[1] + nxy * i1[2], b);
 #pragma unroll
     for (int c = 0; c < 3; ++c) c01[c] = lerp_f64(a[c], b[c], w[0]);
-    node3_f64(f, i0[0] + nx * i1[1] + nxy * i1[2], a);
-    node3_f64(f, i1[0] + nx * i1[1] + nxy * i1[2], b);
+    node3_f64(f, base, i0[0] + nx * i1[1] + nxy * i1[2], a);
+    node3_f64(f, base, i1[0] + nx * i1[1] + nxy * i1[2], b);
 #pragma unroll
     for (int c = 0; c < 3; ++c) c11[c] = lerp_f64(a[c], b[c], w[0]);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
         out[c] = lerp_f64(lerp_f64(c00[c], c10[c], w[1]), lerp_f64(c01[c], c11[c], w[1]), w[2]);
+}
+
+// Time-dependent grids, LazyTimeInterpolator (src/utility/interpolation.jl:
+// 337-430): k = number of snapshot times <= t (searchsortedlast); k == 0
+// clamps to the first snapshot, k == nsnap to the last, otherwise snapshots
+// k-1 and k blend with w = (t - t_{k-1}) / (t_k - t_{k-1})
+__device__ __forceinline__ int time_bracket(const DevField& f, double t, double& w)
+{
+    int lo = 0, hi = f.nsnap;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (f.times[mid] <= t) lo = mid + 1;
+        else hi = mid;
+    }
+    w = 0.0;
+    if (lo > 0 && lo < f.nsnap) w = (t - f.times[lo - 1]) / (f.times[lo] - f.times[lo - 1]);
+    return lo;
+}
+__device__ __forceinline__ const void* snapshot(const DevField& f, int k)
+{
+    return (const char*)f.data + (size_t)k * (size_t)f.snap_bytes;
 }
 
 __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], double t, double o[3])
@@ -342,10 +376,96 @@ __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], 
     case kFieldGrid:
         grid_f64(f, x, o);
         return;
+    case kFieldTimeGrid: {
+        double w;
+        const int k = time_bracket(f, t, w);
+        if (k == 0 || k >= f.nsnap) {
+            grid_f64(f, x, o, snapshot(f, k == 0 ? 0 : f.nsnap - 1));
+            return;
+        }
+        double a[3], b[3];
+        grid_f64(f, x, a, snapshot(f, k - 1));
+        grid_f64(f, x, b, snapshot(f, k));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = (1.0 - w) * a[c] + w * b[c];   // interpolation.jl:377
+        return;
+    }
     default:  // zero
         o[0] = o[1] = o[2] = 0.0;
         return;
     }
+}
+
+// slope of the trilinear interpolant at x (snapshot `data`): J[i][j] = dB_i/dx_j;
+// zero along an axis where a clamped grid is clamped
+__device__ __forceinline__ void grid_jac_f64(const DevField& f, const void* data, const double x[3],
+                                             double J[3][3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) J[i][0] = J[i][1] = J[i][2] = 0.0;
+    // trilinear slope; zero along an axis where a clamped grid is clamped
+    int i0[3], i1[3];
+    double w[3];
+    bool flat[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int nd = f.n[d];
+        double s = (x[d] - f.origin[d]) * f.inv_h[d];
+        const double smax = (double)(nd - 1);
+        flat[d] = false;
+        if (s != s || (f.extrap == kExtrapPeriodic && !(fabs(s) < 4.0e18))) {
+            J[0][0] = __builtin_nan("");
+            return;
+        }
+        if (f.extrap == kExtrapPeriodic) {
+            const double fl = floor(s);
+            w[d] = s - fl;
+            long long ii = (long long)fl % (long long)nd;
+            if (ii < 0) ii += nd;
+            i0[d] = (int)ii;
+            i1[d] = (i0[d] + 1 == nd) ? 0 : i0[d] + 1;
+        } else {
+            if (f.extrap == kExtrapFillNaN) {
+                if (!(s >= 0.0 && s <= smax)) return;     // B is NaN already
+            } else {
+                flat[d] = s < 0.0 || s > smax;
+                s = s < 0.0 ? 0.0 : s;
+                s = s > smax ? smax : s;
+            }
+            double fl = floor(s);
+            if (fl > smax - 1.0) fl = smax - 1.0;
+            w[d] = s - fl;
+            i0[d] = (int)fl;
+            i1[d] = i0[d] + 1;
+        }
+    }
+    const long long nx = f.n[0], nxy = (long long)f.n[0] * f.n[1];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        double c[3];
+        node3_f64(f, data, ((q & 1) ? i1[0] : i0[0]) + nx * (((q >> 1) & 1) ? i1[1] : i0[1]) +
+                         nxy * ((q >> 2) ? i1[2] : i0[2]), c);
+        const double ax = (q & 1) ? w[0] : 1.0 - w[0];
+        const double ay = ((q >> 1) & 1) ? w[1] : 1.0 - w[1];
+        const double az = (q >> 2) ? w[2] : 1.0 - w[2];
+        const double sx = (q & 1) ? 1.0 : -1.0;
+        const double sy = ((q >> 1) & 1) ? 1.0 : -1.0;
+        const double sz = (q >> 2) ? 1.0 : -1.0;
+        const double dwx = (sx * ay) * az, dwy = (ax * sy) * az, dwz = (ax * ay) * sz;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            J[i][0] = J[i][0] + c[i] * dwx;
+            J[i][1] = J[i][1] + c[i] * dwy;
+            J[i][2] = J[i][2] + c[i] * dwz;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double sd = flat[d] ? 0.0 : f.inv_h[d];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) J[i][d] = J[i][d] * sd;
+    }
+    return;
 }
 
 // ---------------------------------------------------------------------------
@@ -393,69 +513,29 @@ __device__ __forceinline__ void field_jac_f64(const DevField& f, const double x[
             for (int j = 0; j < 3; ++j) J[i][j] = a * dg[i][j] - (b * x[j]) * g[i];
         return;
     }
-    case kFieldGrid: {
-        // trilinear slope; zero along an axis where a clamped grid is clamped
-        int i0[3], i1[3];
-        double w[3];
-        bool flat[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int nd = f.n[d];
-            double s = (x[d] - f.origin[d]) * f.inv_h[d];
-            const double smax = (double)(nd - 1);
-            flat[d] = false;
-            if (s != s || (f.extrap == kExtrapPeriodic && !(fabs(s) < 4.0e18))) {
-                J[0][0] = __builtin_nan("");
-                return;
-            }
-            if (f.extrap == kExtrapPeriodic) {
-                const double fl = floor(s);
-                w[d] = s - fl;
-                long long ii = (long long)fl % (long long)nd;
-                if (ii < 0) ii += nd;
-                i0[d] = (int)ii;
-                i1[d] = (i0[d] + 1 == nd) ? 0 : i0[d] + 1;
-            } else {
-                if (f.extrap == kExtrapFillNaN) {
-                    if (!(s >= 0.0 && s <= smax)) return;     // B is NaN already
-                } else {
-                    flat[d] = s < 0.0 || s > smax;
-                    s = s < 0.0 ? 0.0 : s;
-                    s = s > smax ? smax : s;
-                }
-                double fl = floor(s);
-                if (fl > smax - 1.0) fl = smax - 1.0;
-                w[d] = s - fl;
-                i0[d] = (int)fl;
-                i1[d] = i0[d] + 1;
-            }
+    case kFieldGrid:
+        grid_jac_f64(f, f.data, x, J);
+        return;
+    case kFieldTimeGrid: {
+        // jacobian and derivative_t of LazyTimeInterpolator (interpolation.jl:396-430)
+        double w;
+        const int k = time_bracket(f, t, w);
+        if (k == 0 || k >= f.nsnap) {
+            grid_jac_f64(f, snapshot(f, k == 0 ? 0 : f.nsnap - 1), x, J);
+            return;
         }
-        const long long nx = f.n[0], nxy = (long long)f.n[0] * f.n[1];
+        double J1[3][3], J2[3][3], a[3], b[3];
+        grid_jac_f64(f, snapshot(f, k - 1), x, J1);
+        grid_jac_f64(f, snapshot(f, k), x, J2);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            double c[3];
-            node3_f64(f, ((q & 1) ? i1[0] : i0[0]) + nx * (((q >> 1) & 1) ? i1[1] : i0[1]) +
-                             nxy * ((q >> 2) ? i1[2] : i0[2]), c);
-            const double ax = (q & 1) ? w[0] : 1.0 - w[0];
-            const double ay = ((q >> 1) & 1) ? w[1] : 1.0 - w[1];
-            const double az = (q >> 2) ? w[2] : 1.0 - w[2];
-            const double sx = (q & 1) ? 1.0 : -1.0;
-            const double sy = ((q >> 1) & 1) ? 1.0 : -1.0;
-            const double sz = (q >> 2) ? 1.0 : -1.0;
-            const double dwx = (sx * ay) * az, dwy = (ax * sy) * az, dwz = (ax * ay) * sz;
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                J[i][0] = J[i][0] + c[i] * dwx;
-                J[i][1] = J[i][1] + c[i] * dwy;
-                J[i][2] = J[i][2] + c[i] * dwz;
-            }
-        }
+            for (int j = 0; j < 3; ++j) J[i][j] = (1.0 - w) * J1[i][j] + w * J2[i][j];
+        grid_f64(f, x, a, snapshot(f, k - 1));
+        grid_f64(f, x, b, snapshot(f, k));
+        const double dtk = f.times[k] - f.times[k - 1];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double sd = flat[d] ? 0.0 : f.inv_h[d];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) J[i][d] = J[i][d] * sd;
-        }
+        for (int i = 0; i < 3; ++i) dBdt[i] = (b[i] - a[i]) / dtk;
         return;
     }
     default:  // zero, uniform
@@ -643,7 +723,8 @@ __device__ __forceinline__ void node3_f32(const T* __restrict__ d, long long idx
 // Trilinear gather in float from frame-relative coordinates (frame origin =
 // the grid origin, so s = (hi + lo) / h without cancellation).
 template <typename T>
-__device__ __forceinline__ void grid_f32_t(const DevField& f, const Pos32& x, float out[3])
+__device__ __forceinline__ void grid_f32_t(const DevField& f, const Pos32& x, float out[3],
+                                           const void* data = nullptr)
 {
     int i0[3], i1[3];
     float w[3];
@@ -679,7 +760,7 @@ __device__ __forceinline__ void grid_f32_t(const DevField& f, const Pos32& x, fl
             i1[d] = i0[d] + 1;
         }
     }
-    const T* __restrict__ d = (const T*)f.data;
+    const T* __restrict__ d = (const T*)(data ? data : f.data);
     const long long nx = f.n[0], nxy = (long long)f.n[0] * f.n[1];
     float a[3], b[3], c00[3], c10[3], c01[3], c11[3];
     node3_f32(d, i0[0] + nx * i0[1] + nxy * i0[2], a);
@@ -723,6 +804,27 @@ __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, c
         if (f.dtype == 1) grid_f32_t<float>(f, x, o);
         else grid_f32_t<double>(f, x, o);
         return;
+    case kFieldTimeGrid: {
+        double w;
+        const int k = time_bracket(f, t, w);
+        const int k0 = k == 0 ? 0 : k - 1, k1 = k >= f.nsnap ? f.nsnap - 1 : k;
+        float a[3], b[3];
+        if (f.dtype == 1) {
+            grid_f32_t<float>(f, x, a, snapshot(f, k0));
+            if (k1 != k0) grid_f32_t<float>(f, x, b, snapshot(f, k1));
+        } else {
+            grid_f32_t<double>(f, x, a, snapshot(f, k0));
+            if (k1 != k0) grid_f32_t<double>(f, x, b, snapshot(f, k1));
+        }
+        if (k == 0 || k >= f.nsnap) {
+            o[0] = a[0]; o[1] = a[1]; o[2] = a[2];
+            return;
+        }
+        const float wf = (float)w;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = fmaf(wf, b[c], (1.0f - wf) * a[c]);
+        return;
+    }
     case kFieldAffine: {
         double xa[3];
         pos_abs(p, x, xa);

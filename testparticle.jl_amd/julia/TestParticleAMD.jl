@@ -111,6 +111,16 @@ struct UxBDeviceField <: DeviceField
     u::SVector{3, Float64}
     B::DeviceField
 end
+"""
+Time-dependent gridded field, the device form of
+`LazyTimeInterpolator(times, loader)` (src/utility/interpolation.jl:337-430):
+`loader(i)` returns the `GridDeviceField` of snapshot i on one shared grid.
+Every snapshot is loaded once and kept resident on the device.
+"""
+struct TimeGridDeviceField <: DeviceField
+    times::Vector{Float64}
+    loader::Function
+end
 
 handle(ctx, ::ZeroDeviceField, cache) = _mk(ctx) do h
     ccall((:tpamd_field_zero, libtpamd), Int32, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}), ctx, h)
@@ -133,6 +143,26 @@ function handle(ctx, f::GridDeviceField{T}, cache) where {T}
                  Int32, Int32, Ptr{Ptr{Cvoid}}),
                 ctx, f.A, T === Float64 ? 0 : 1, size(f.A, 2), size(f.A, 3), size(f.A, 4),
                 origin, spacing, f.extrap, f.storage, h)
+        end
+    end
+end
+function handle(ctx, f::TimeGridDeviceField, cache)
+    get!(cache, f) do
+        snaps = [f.loader(i)::GridDeviceField for i in eachindex(f.times)]
+        g = snaps[1]
+        T = eltype(g.A)
+        all(s -> size(s.A) == size(g.A) && eltype(s.A) === T && s.x == g.x && s.y == g.y &&
+                 s.z == g.z && s.extrap == g.extrap && s.storage == g.storage, snaps) ||
+            throw(ArgumentError("every snapshot must share the grid of snapshot 1"))
+        _mk(ctx) do h
+            origin = [first(g.x), first(g.y), first(g.z)]
+            spacing = [step(g.x), step(g.y), step(g.z)]
+            ptrs = [pointer(s.A) for s in snaps]
+            GC.@preserve snaps ccall((:tpamd_field_time_grid, libtpamd), Int32,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Int32, Int64, Int64, Int64, Int32, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Int32, Int32, Ptr{Ptr{Cvoid}}),
+                ctx, ptrs, T === Float64 ? 0 : 1, size(g.A, 2), size(g.A, 3), size(g.A, 4),
+                length(f.times), f.times, origin, spacing, g.extrap, g.storage, h)
         end
     end
 end

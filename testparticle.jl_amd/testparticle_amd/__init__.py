@@ -17,8 +17,8 @@ from .algorithms import (AdaptiveBoris, AdaptiveMultistepBoris, Boris, EnsembleD
 from .constants import (BMoment_Earth, Electron, Ion, Proton, Re, Species, SpeciesDict, c, eV,
                         get_gyrofrequency, get_gyroperiod, kB, me, mi, qe, qi)
 from .fields import (AffineField, ClampExtrap, DipoleField, FieldInterpolator, FillExtrap,
-                     GridField, PeriodicBC, UniformField, UxBField, WrapExtrap, ZeroField,
-                     build_interpolator, prepare)
+                     GridField, LazyTimeInterpolator, PeriodicBC, UniformField, UxBField,
+                     WrapExtrap, ZeroField, build_interpolator, prepare)
 from .problem import (Boundary, EnsembleContext, InsideSphere, OutsideBox, OutsideGrid,
                       OutsideNaN, OutsideSphere, TerminateOutside, TraceProblem, remake)
 from .solution import EnsembleSolution, TraceSolution

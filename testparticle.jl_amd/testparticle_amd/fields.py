@@ -214,6 +214,71 @@ class GridField(AbstractField):
 FieldInterpolator = GridField
 
 
+class LazyTimeInterpolator(AbstractField):
+    """Linear interpolation in time between gridded snapshots
+    (LazyTimeInterpolator(times, loader), src/utility/interpolation.jl:
+    337-430).  `loader(i)` (1-based, as in Julia) returns the GridField of
+    snapshot i; every snapshot must share the first one's grid, boundary
+    condition and storage.  At time t, with k = searchsortedlast(times, t):
+    k == 0 gives snapshot 1, k == len(times) the last snapshot, otherwise
+    (1 - w) A_k(x) + w A_{k+1}(x), w = (t - t_k) / (t_{k+1} - t_k).
+
+    The reference loads snapshots on demand and keeps at most three; here the
+    first upload to a device calls the loader for every snapshot once and
+    keeps them all resident in device memory, which the kernels read at any
+    t without a host round trip.  The loader is called for snapshot 1 at
+    construction, like the reference's constructor."""
+
+    def __init__(self, times, loader):
+        super().__init__()
+        t = np.asarray(times, dtype=np.float64).reshape(-1).copy()
+        if t.size < 1:
+            raise _lib.ArgumentError("LazyTimeInterpolator needs at least one time")
+        if not np.all(np.isfinite(t)):
+            raise _lib.ArgumentError("snapshot times must be finite")
+        if np.any(np.diff(t) <= 0):
+            raise _lib.ArgumentError("snapshot times must be strictly increasing")
+        self.times = t
+        self.loader = loader
+        self.time_dependent = t.size > 1
+        first = self._load(1)
+        self.shape, self.extrap, self.storage = first.shape, first.extrap, first.storage
+        self.origin, self.spacing = first.origin.copy(), first.spacing.copy()
+        self._first = first
+
+    def _load(self, i: int) -> GridField:
+        f = self.loader(i)
+        if not isinstance(f, GridField):
+            raise TypeError("the device path needs the loader to return gridded fields "
+                            "(build_interpolator / GridField)")
+        return f
+
+    def snapshots(self):
+        """Yields (index, GridField) for every snapshot, 1-based, checking that
+        each shares the first snapshot's grid."""
+        for i in range(1, self.times.size + 1):
+            f = self._first if i == 1 else self._load(i)
+            if (f.shape != self.shape or f.extrap != self.extrap or f.storage != self.storage
+                    or not np.array_equal(f.origin, self.origin)
+                    or not np.array_equal(f.spacing, self.spacing)):
+                raise _lib.ArgumentError(f"snapshot {i} does not share the grid of snapshot 1")
+            yield i, f
+
+    def _create(self, ctx, h):
+        datas = []
+        for _, f in self.snapshots():
+            d = f.data
+            if d.dtype != self._first.data.dtype:
+                d = d.astype(self._first.data.dtype)
+            datas.append(d)
+        ptrs = (C.c_void_p * len(datas))(*[d.ctypes.data for d in datas])
+        dt = _lib.F64 if datas[0].dtype == np.float64 else _lib.F32
+        st = _lib.F64 if self.storage == "f64" else _lib.F32
+        return ctx.L.tpamd_field_time_grid(ctx.h, ptrs, dt, *self.shape, len(datas),
+                                           _lib.dptr(self.times), _lib.dptr(self.origin),
+                                           _lib.dptr(self.spacing), self.extrap, st, C.byref(h))
+
+
 class UxBField(AbstractField):
     """E = -u x B(x) for a uniform drift velocity u (ideal-MHD electric field)."""
 

@@ -29,6 +29,10 @@ def oracle_field(f, B_oracle=None):
         return O.grid(data, f.origin, f.spacing, f.extrap)
     if isinstance(f, TP.UxBField):
         return O.uxb(f.u, B_oracle)
+    if isinstance(f, TP.LazyTimeInterpolator):
+        dt = np.float32 if f.storage == "f32" else np.float64
+        snaps = np.stack([g.data.astype(dt, copy=False) for _, g in f.snapshots()])
+        return O.time_grid(snaps, f.times, f.origin, f.spacing, f.extrap)
     raise TypeError(type(f))
 
 
@@ -80,9 +84,14 @@ def assert_bitwise(gpu: TP.EnsembleSolution, orc: O.OracleResult, n_check=None):
         n = min(int(gpu.lengths[i]), cap)
         gu, ou = gpu.u_all[i, :n], orc.u[i, :n]
         gt, ot = gpu.t_all[i, :n], orc.t[i, :n]
-        if not (np.array_equal(gu, ou, equal_nan=True) and np.array_equal(gt, ot)):
+        # a record holds NaN where the field was NaN (FillExtrap outside the
+        # grid); the adaptive step then turns t into NaN too, on both sides
+        if not (np.array_equal(gu, ou, equal_nan=True) and np.array_equal(gt, ot, equal_nan=True)):
             d = np.nanmax(np.abs(gu - ou) / np.maximum(np.abs(ou), 1e-300))
-            raise AssertionError(f"trajectory {i}: GPU and oracle differ (max rel {d:.3e})")
+            nan_u = int(np.sum(np.isnan(gu) != np.isnan(ou)))
+            dt = np.nanmax(np.abs(gt - ot)) if gt.size else 0.0
+            raise AssertionError(f"trajectory {i}: GPU and oracle differ (max rel {d:.3e}, "
+                                 f"{nan_u} NaN mismatches, max |dt| {dt:.3e})")
 
 
 def rel_err(a, b, floor=0.0):
