@@ -53,14 +53,16 @@ typedef enum tpamd_field_kind {
     TPAMD_FIELD_DIPOLE = 3,   /* B = b0 R^3/r^5 (-3xz, -3yz, r^2-3z^2) */
     TPAMD_FIELD_GRID = 4,     /* FieldInterpolator over a uniform CartesianGrid, order 1 */
     TPAMD_FIELD_UXB = 5,      /* E = -u x B(x), B the solve's magnetic field */
-    TPAMD_FIELD_TIME_GRID = 6 /* LazyTimeInterpolator over gridded snapshots */
+    TPAMD_FIELD_TIME_GRID = 6,/* LazyTimeInterpolator over gridded snapshots */
+    TPAMD_FIELD_MESH = 7      /* rectilinear / cubic / spherical / 2D / 1D grids */
 } tpamd_field_kind;
 
 /* Extrapolation of gridded fields, src/utility/interpolation.jl:203-206 */
 typedef enum tpamd_extrap {
     TPAMD_EXTRAP_FILL_NAN = 0, /* FillExtrap(NaN), the prepare() default (src/prepare.jl:257-265) */
     TPAMD_EXTRAP_CLAMP = 1,    /* ClampExtrap() */
-    TPAMD_EXTRAP_PERIODIC = 2  /* WrapExtrap() / PeriodicBC(endpoint=:exclusive): period N*h */
+    TPAMD_EXTRAP_PERIODIC = 2, /* WrapExtrap() / PeriodicBC(endpoint=:exclusive): period N*h */
+    TPAMD_EXTRAP_PERIODIC_INCLUSIVE = 3 /* PeriodicBC(): period x_N - x_1 (meshes only) */
 } tpamd_extrap;
 
 typedef enum tpamd_dtype { TPAMD_F64 = 0, TPAMD_F32 = 1 } tpamd_dtype;
@@ -193,6 +195,24 @@ int32_t tpamd_field_time_grid(tpamd_ctx* ctx, const void* const* snapshots, int3
                               const double* times, const double origin[3],
                               const double spacing[3], int32_t extrap, int32_t storage_dtype,
                               tpamd_field** out);
+/* Gridded field on a general mesh, the device counterpart of
+ * build_interpolator(gridtype, A, g1, g2, g3, order, bc)
+ * (src/utility/interpolation.jl:15-117,140-178,217-319):
+ *   coord = TPAMD_MESH_CARTESIAN: CartesianGrid / RectilinearGrid with any
+ *     strictly increasing axes (order 1; order 3 on uniform axes); an axis of
+ *     one node is ignored, which gives the 2D and 1D interpolators;
+ *   coord = TPAMD_MESH_SPHERICAL: StructuredGrid with axes (r, theta, phi),
+ *     values (B_r, B_theta, B_phi) rotated to Cartesian with sph2cartvec.
+ * `data` is laid out as for tpamd_field_grid with n[0] x n[1] x n[2] nodes;
+ * axes[d] holds the n[d] node coordinates of axis d; extrap[d] is that axis's
+ * boundary (TPAMD_EXTRAP_*; the exclusive periodic wrap needs a uniform axis).
+ * Order 3 is the cardinal cubic: Catmull-Rom weights on nodes i-1 .. i+2,
+ * with the linear extrapolation 2 f_0 - f_1 standing in for a missing
+ * neighbour at a non-periodic end. */
+enum { TPAMD_MESH_CARTESIAN = 0, TPAMD_MESH_SPHERICAL = 1 };
+int32_t tpamd_field_mesh(tpamd_ctx* ctx, const void* data, int32_t dtype, const int64_t n[3],
+                         const double* const axes[3], int32_t coord, int32_t order,
+                         const int32_t extrap[3], int32_t storage_dtype, tpamd_field** out);
 /* Builds (enable = 1) or drops (0) a cell-replicated copy of a gridded field:
  * per cell the 8 corner nodes x 3 components in one block (float: 128 B, one
  * cache line; double: 192 B), so a trilinear gather reads one contiguous

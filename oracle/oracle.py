@@ -18,7 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libtporacle.so")
 
 FIELD_ZERO, FIELD_UNIFORM, FIELD_AFFINE, FIELD_DIPOLE, FIELD_GRID, FIELD_UXB = 0, 1, 2, 3, 4, 5
-FIELD_TIME_GRID = 6
+FIELD_TIME_GRID, FIELD_MESH = 6, 7
+EXTRAP_PERIODIC_INCL = 3
+MESH_CARTESIAN, MESH_SPHERICAL = 0, 1
 FIELD_KAT_FRONT, FIELD_KAT_SWITCH_ON, FIELD_KAT_NAN_OUTSIDE = 100, 101, 102
 EXTRAP_FILL_NAN, EXTRAP_CLAMP, EXTRAP_PERIODIC = 0, 1, 2
 ALG_BORIS, ALG_MULTISTEP = 0, 1
@@ -40,6 +42,8 @@ OrcField._fields_ = [
     ("origin", C.c_double * 3), ("spacing", C.c_double * 3),
     ("src", C.POINTER(OrcField)),
     ("times", C.POINTER(C.c_double)), ("nsnap", C.c_long),
+    ("axes", C.POINTER(C.c_double) * 3), ("coord", C.c_int), ("order", C.c_int),
+    ("mesh_extrap", C.c_int * 3), ("pad2_", C.c_int),
 ]
 
 
@@ -158,6 +162,24 @@ def time_grid(snaps: np.ndarray, times, origin, spacing, extrap: int) -> Field:
     st.origin[:] = [float(a) for a in origin]
     st.spacing[:] = [float(a) for a in spacing]
     return Field(st, keep=(snaps, tm))
+
+
+def mesh(data: np.ndarray, axes, coord: int, order: int, extrap) -> Field:
+    """General mesh (rectilinear / cubic / spherical / 2D / 1D).  data:
+    (nz, ny, nx, 3) C order; axes: the three node-coordinate vectors (length
+    1 for an ignored axis); extrap: boundary per axis."""
+    assert data.ndim == 4 and data.shape[3] == 3
+    assert data.dtype in (np.float64, np.float32)
+    data = np.ascontiguousarray(data)
+    nz, ny, nx, _ = data.shape
+    ax = [np.ascontiguousarray(a, dtype=np.float64) for a in axes]
+    assert [len(a) for a in ax] == [nx, ny, nz]
+    st = OrcField(kind=FIELD_MESH, dtype=0 if data.dtype == np.float64 else 1,
+                  data=data.ctypes.data, nx=nx, ny=ny, nz=nz, coord=coord, order=order)
+    for d in range(3):
+        st.axes[d] = _dp(ax[d])
+        st.mesh_extrap[d] = int(extrap[d])
+    return Field(st, keep=(data, ax))
 
 
 def uxb(u, B: Field) -> Field:

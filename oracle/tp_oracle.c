@@ -309,6 +309,309 @@ static void time_grid_eval(const orc_field* f, const double x[3], double t, doub
     for (int c = 0; c < 3; ++c) out[c] = (1.0 - w) * a[c] + w * b[c];     /* :378 */
 }
 
+/* ----------------------------------------------------------------------------
+ * General meshes, build_interpolator(gridtype, A, g1, g2, g3, order, bc)
+ * (src/utility/interpolation.jl:15-117,140-178,217-319), restated:
+ *   - an axis is treated as uniform when every node lies within 1e-9 of its
+ *     span of c_1 + (i-1) h (the library's choice: a uniform axis locates by
+ *     s = (x - c_1) / h as the uniform grid does, any other by search);
+ *   - a non-uniform axis finds by bisection the largest i in [0, n-2] with
+ *     c[i] <= x, and t = (x - c[i]) / (c[i+1] - c[i]);
+ *   - an axis of one node is ignored (the 2D and 1D interpolators);
+ *   - periodic inclusive wraps x into [c_1, c_N] with period c_N - c_1;
+ *   - order 1 blends x, then y, then z with lerp; order 3 is the cardinal
+ *     (Catmull-Rom) cubic, pinned by test/test_utility.jl:113-121, with the
+ *     linear extrapolation 2 f_1 - f_2 for a neighbour past a non-periodic
+ *     end;
+ *   - spherical meshes: (r, theta, phi) = cart2sph(x) (utility.jl:63-71),
+ *     result rotated by sph2cartvec (utility.jl:82-89).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    long i;          /* cell */
+    double t;        /* fraction in the cell */
+    double dtdx;     /* d t / d (coordinate); 0 where clamped or ignored */
+} mesh_loc;
+
+static int mesh_axis_uniform(const double* c, long n)
+{
+    const double lo = c[0], hi = c[n - 1], h = (hi - lo) / (double)(n - 1);
+    for (long i = 1; i < n; ++i)
+        if (fabs(c[i] - (lo + (double)i * h)) > 1e-9 * (hi - lo)) return 0;
+    return 1;
+}
+
+/* 0 where the field is NaN (FillExtrap outside, NaN coordinate) */
+static int mesh_axis_locate(const double* c, long n, int bc, double x, mesh_loc* L)
+{
+    L->i = 0; L->t = 0.0; L->dtdx = 0.0;
+    if (n == 1) return 1;
+    if (isnan(x)) return 0;
+    const double lo = c[0], hi = c[n - 1];
+    if (bc == ORC_EXTRAP_PERIODIC_INCL) {
+        const double period = hi - lo;
+        double u = x - lo;
+        u = u - period * floor(u / period);
+        x = lo + u;
+        if (x > hi) x = hi;
+    }
+    int flat = 0;
+    if (mesh_axis_uniform(c, n)) {
+        const double h = (hi - lo) / (double)(n - 1), inv_h = 1.0 / h;
+        double s = (x - lo) * inv_h;
+        const double smax = (double)(n - 1);
+        if (bc == ORC_EXTRAP_PERIODIC) {
+            if (!(fabs(s) < 4.0e18)) return 0;
+            const double fl = floor(s);
+            long ii = (long)fl % n;
+            if (ii < 0) ii += n;
+            L->i = ii; L->t = s - fl; L->dtdx = inv_h;
+            return 1;
+        }
+        if (bc == ORC_EXTRAP_FILL_NAN) {
+            if (!(s >= 0.0 && s <= smax)) return 0;
+        } else {
+            flat = bc == ORC_EXTRAP_CLAMP && (s < 0.0 || s > smax);
+            if (s < 0.0) s = 0.0;
+            if (s > smax) s = smax;
+        }
+        double fl = floor(s);
+        if (fl > smax - 1.0) fl = smax - 1.0;
+        L->i = (long)fl; L->t = s - fl; L->dtdx = flat ? 0.0 : inv_h;
+        return 1;
+    }
+    if (bc == ORC_EXTRAP_FILL_NAN) {
+        if (!(x >= lo && x <= hi)) return 0;
+    } else if (bc == ORC_EXTRAP_CLAMP) {
+        flat = x < lo || x > hi;
+        if (x < lo) x = lo;
+        if (x > hi) x = hi;
+    }
+    long a = 0, b = n - 2;          /* bisection: largest i with c[i] <= x */
+    while (a < b) {
+        const long m = (a + b + 1) / 2;
+        if (c[m] <= x) a = m;
+        else b = m - 1;
+    }
+    const double h = c[a + 1] - c[a];
+    L->i = a; L->t = (x - c[a]) / h; L->dtdx = flat ? 0.0 : 1.0 / h;
+    return 1;
+}
+
+static long mesh_index(long n, int bc, long j)
+{
+    if (n == 1) return 0;
+    if (bc == ORC_EXTRAP_PERIODIC) {
+        j %= n;
+        return j < 0 ? j + n : j;
+    }
+    if (bc == ORC_EXTRAP_PERIODIC_INCL) {
+        if (j >= 0 && j <= n - 1) return j;
+        const long m = n - 1;
+        j %= m;
+        return j < 0 ? j + m : j;
+    }
+    return j < 0 ? 0 : (j > n - 1 ? n - 1 : j);
+}
+
+/* stencil nodes and weights along one axis: order 1 uses entries 0, 1 */
+static void mesh_weights(const orc_field* f, int d, const mesh_loc* L, long idx[4], double w[4],
+                         double dw[4])
+{
+    const long n = d == 0 ? f->nx : (d == 1 ? f->ny : f->nz);
+    const int bc = f->mesh_extrap[d];
+    if (n == 1) {
+        for (int k = 0; k < 4; ++k) { idx[k] = 0; w[k] = 0.0; dw[k] = 0.0; }
+        if (f->order == 3) w[1] = 1.0;
+        return;
+    }
+    const double t = L->t;
+    if (f->order == 1) {
+        idx[0] = L->i;
+        idx[1] = mesh_index(n, bc, L->i + 1);
+        w[0] = 1.0 - t; w[1] = t; dw[0] = -1.0; dw[1] = 1.0;
+        return;
+    }
+    /* Catmull-Rom: p(t) = sum_k w_k f_{i-1+k} */
+    w[0] = ((-t + 2.0) * t - 1.0) * t * 0.5;
+    w[1] = ((3.0 * t - 5.0) * t * t + 2.0) * 0.5;
+    w[2] = ((-3.0 * t + 4.0) * t + 1.0) * t * 0.5;
+    w[3] = (t - 1.0) * t * t * 0.5;
+    dw[0] = ((-3.0 * t + 4.0) * t - 1.0) * 0.5;
+    dw[1] = (9.0 * t - 10.0) * t * 0.5;
+    dw[2] = ((-9.0 * t + 8.0) * t + 1.0) * 0.5;
+    dw[3] = (3.0 * t - 2.0) * t * 0.5;
+    const int per = bc == ORC_EXTRAP_PERIODIC || bc == ORC_EXTRAP_PERIODIC_INCL;
+    if (!per && L->i == 0) {
+        w[1] = w[1] + 2.0 * w[0]; w[2] = w[2] - w[0];
+        dw[1] = dw[1] + 2.0 * dw[0]; dw[2] = dw[2] - dw[0];
+        w[0] = 0.0; dw[0] = 0.0;
+    }
+    if (!per && L->i + 2 > n - 1) {
+        w[2] = w[2] + 2.0 * w[3]; w[1] = w[1] - w[3];
+        dw[2] = dw[2] + 2.0 * dw[3]; dw[1] = dw[1] - dw[3];
+        w[3] = 0.0; dw[3] = 0.0;
+    }
+    for (int k = 0; k < 4; ++k) idx[k] = mesh_index(n, bc, L->i - 1 + k);
+}
+
+static inline double mesh_node(const orc_field* f, long i, long j, long k, int c)
+{
+    return grid_node(f, i, j, k, c);
+}
+
+/* interpolant at mesh coordinates q; J[3a+j] = dF_a/dq_j when J != NULL */
+static int mesh_interp(const orc_field* f, const double q[3], double out[3], double* J)
+{
+    mesh_loc L[3];
+    const double* ax[3] = {f->axes[0], f->axes[1], f->axes[2]};
+    const long n[3] = {f->nx, f->ny, f->nz};
+    for (int d = 0; d < 3; ++d)
+        if (!mesh_axis_locate(ax[d], n[d], f->mesh_extrap[d], q[d], &L[d])) return 0;
+    long ix[4], iy[4], iz[4];
+    double wx[4], wy[4], wz[4], dwx[4], dwy[4], dwz[4];
+    mesh_weights(f, 0, &L[0], ix, wx, dwx);
+    mesh_weights(f, 1, &L[1], iy, wy, dwy);
+    mesh_weights(f, 2, &L[2], iz, wz, dwz);
+    if (f->order == 1) {
+        for (int c = 0; c < 3; ++c) {
+            const double tx = L[0].t, ty = L[1].t, tz = L[2].t;
+            const double c00 = lerp(mesh_node(f, ix[0], iy[0], iz[0], c), mesh_node(f, ix[1], iy[0], iz[0], c), tx);
+            const double c10 = lerp(mesh_node(f, ix[0], iy[1], iz[0], c), mesh_node(f, ix[1], iy[1], iz[0], c), tx);
+            const double c01 = lerp(mesh_node(f, ix[0], iy[0], iz[1], c), mesh_node(f, ix[1], iy[0], iz[1], c), tx);
+            const double c11 = lerp(mesh_node(f, ix[0], iy[1], iz[1], c), mesh_node(f, ix[1], iy[1], iz[1], c), tx);
+            out[c] = lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
+        }
+        if (J) {
+            for (int i = 0; i < 9; ++i) J[i] = 0.0;
+            /* corner sum of the trilinear slope, corners in x-fastest order */
+            for (int k = 0; k < 8; ++k) {
+                const int bx = k & 1, by = (k >> 1) & 1, bz = k >> 2;
+                const double ax_ = bx ? L[0].t : 1.0 - L[0].t;
+                const double ay = by ? L[1].t : 1.0 - L[1].t;
+                const double az = bz ? L[2].t : 1.0 - L[2].t;
+                const double sx = bx ? 1.0 : -1.0, sy = by ? 1.0 : -1.0, sz = bz ? 1.0 : -1.0;
+                const double gx = (sx * ay) * az, gy = (ax_ * sy) * az, gz = (ax_ * ay) * sz;
+                for (int a = 0; a < 3; ++a) {
+                    const double v = mesh_node(f, ix[bx], iy[by], iz[bz], a);
+                    J[3 * a] = J[3 * a] + v * gx;
+                    J[3 * a + 1] = J[3 * a + 1] + v * gy;
+                    J[3 * a + 2] = J[3 * a + 2] + v * gz;
+                }
+            }
+            for (int d = 0; d < 3; ++d)
+                for (int a = 0; a < 3; ++a) J[3 * a + d] = J[3 * a + d] * L[d].dtdx;
+        }
+        return 1;
+    }
+    double v[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+    for (int kz = 0; kz < 4; ++kz) {
+        double vy[3] = {0, 0, 0}, gxy[3] = {0, 0, 0}, gyy[3] = {0, 0, 0};
+        for (int ky = 0; ky < 4; ++ky) {
+            double vx[3] = {0, 0, 0}, gxx[3] = {0, 0, 0};
+            for (int kx = 0; kx < 4; ++kx)
+                for (int a = 0; a < 3; ++a) {
+                    const double fv = mesh_node(f, ix[kx], iy[ky], iz[kz], a);
+                    vx[a] = vx[a] + wx[kx] * fv;
+                    gxx[a] = gxx[a] + dwx[kx] * fv;
+                }
+            for (int a = 0; a < 3; ++a) {
+                vy[a] = vy[a] + wy[ky] * vx[a];
+                gxy[a] = gxy[a] + wy[ky] * gxx[a];
+                gyy[a] = gyy[a] + dwy[ky] * vx[a];
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            v[a] = v[a] + wz[kz] * vy[a];
+            gx[a] = gx[a] + wz[kz] * gxy[a];
+            gy[a] = gy[a] + wz[kz] * gyy[a];
+            gz[a] = gz[a] + dwz[kz] * vy[a];
+        }
+    }
+    for (int a = 0; a < 3; ++a) out[a] = v[a];
+    if (J)
+        for (int a = 0; a < 3; ++a) {
+            J[3 * a] = gx[a] * L[0].dtdx;
+            J[3 * a + 1] = gy[a] * L[1].dtdx;
+            J[3 * a + 2] = gz[a] * L[2].dtdx;
+        }
+    return 1;
+}
+
+/* cart2sph, utility.jl:63-71, with Julia's n-argument hypot */
+static void mesh_cart2sph(const double x[3], double q[3])
+{
+    const double m = fmax(fmax(fabs(x[0]), fabs(x[1])), fabs(x[2]));
+    if (m == 0.0) { q[0] = q[1] = q[2] = 0.0; return; }
+    double r;
+    if (isnan(m) || isinf(m)) {
+        r = (isinf(x[0]) || isinf(x[1]) || isinf(x[2])) ? INFINITY : m;
+    } else {
+        const double a = x[0] / m, b = x[1] / m, c = x[2] / m;
+        r = m * sqrt((a * a + b * b) + c * c);
+    }
+    q[0] = r;
+    q[1] = acos(x[2] / r);
+    double ph = atan2(x[1], x[0]);
+    if (ph < 0.0) {                      /* mod2pi: ph + 2 pi, 2 pi as hi + lo */
+        const double hi = 6.283185307179586, lo = 2.4492935982947064e-16;
+        const double s = hi + ph, bb = s - hi;
+        const double err = (hi - (s - bb)) + (ph - bb);
+        ph = s + (err + lo);
+    }
+    q[2] = ph;
+}
+
+static void mesh_eval(const orc_field* f, const double x[3], double out[3], double* J)
+{
+    if (f->coord == ORC_MESH_CARTESIAN) {
+        if (!mesh_interp(f, x, out, J)) {
+            out[0] = out[1] = out[2] = NAN;
+            if (J) for (int i = 0; i < 9; ++i) J[i] = 0.0;
+        }
+        return;
+    }
+    double q[3], s[3], Js[9];
+    mesh_cart2sph(x, q);
+    if (!mesh_interp(f, q, s, J ? Js : NULL)) {
+        out[0] = out[1] = out[2] = NAN;
+        if (J) for (int i = 0; i < 9; ++i) J[i] = 0.0;
+        return;
+    }
+    const double st = sin(q[1]), ct = cos(q[1]), sp = sin(q[2]), cp = cos(q[2]);
+    out[0] = (st * cp * s[0] + ct * cp * s[1]) - sp * s[2];     /* sph2cartvec */
+    out[1] = (st * sp * s[0] + ct * sp * s[1]) + cp * s[2];
+    out[2] = ct * s[0] - st * s[1];
+    if (!J) return;
+    /* chain rule: F = R(theta, phi) S(r, theta, phi) */
+    const double r = q[0], rho = sqrt(x[0] * x[0] + x[1] * x[1]);
+    double dq[9];
+    for (int j = 0; j < 3; ++j) dq[j] = x[j] / r;
+    if (rho > 0.0) {
+        const double r2 = r * r;
+        dq[3] = (x[2] * x[0]) / (r2 * rho);
+        dq[4] = (x[2] * x[1]) / (r2 * rho);
+        dq[5] = -rho / r2;
+        dq[6] = -x[1] / (rho * rho);
+        dq[7] = x[0] / (rho * rho);
+        dq[8] = 0.0;
+    } else {
+        for (int j = 3; j < 9; ++j) dq[j] = NAN;
+    }
+    const double R[9] = {st * cp, ct * cp, -sp, st * sp, ct * sp, cp, ct, -st, 0.0};
+    const double dRt[9] = {ct * cp, -st * cp, 0.0, ct * sp, -st * sp, 0.0, -st, -ct, 0.0};
+    const double dRp[9] = {-st * sp, -ct * sp, -cp, st * cp, ct * cp, -sp, 0.0, 0.0, 0.0};
+    for (int a = 0; a < 3; ++a) {
+        double rs = 0.0, ts = 0.0, ps = 0.0;
+        for (int b = 0; b < 3; ++b) {
+            rs = rs + R[3 * a + b] * Js[3 * b];
+            ts = ts + (dRt[3 * a + b] * s[b] + R[3 * a + b] * Js[3 * b + 1]);
+            ps = ps + (dRp[3 * a + b] * s[b] + R[3 * a + b] * Js[3 * b + 2]);
+        }
+        for (int j = 0; j < 3; ++j)
+            J[3 * a + j] = (rs * dq[j] + ts * dq[3 + j]) + ps * dq[6 + j];
+    }
+}
+
 void orc_field_eval(const orc_field* f, const double x[3], double t, double out[3])
 {
     switch (f->kind) {
@@ -342,6 +645,9 @@ void orc_field_eval(const orc_field* f, const double x[3], double t, double out[
         return;
     case ORC_FIELD_TIME_GRID:
         time_grid_eval(f, x, t, out);
+        return;
+    case ORC_FIELD_MESH:
+        mesh_eval(f, x, out, NULL);
         return;
     case ORC_FIELD_UXB: {                      /* E = -u x B = B x u */
         double B[3];
@@ -403,6 +709,11 @@ void orc_field_jac(const orc_field* f, const double x[3], double t, double B[3],
                               2.0 * x[0], 2.0 * x[1], -4.0 * x[2]};
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) J[3 * i + j] = a * dg[3 * i + j] - (b * x[j]) * g[i];
+        return;
+    }
+    case ORC_FIELD_MESH: {
+        double v[3];
+        mesh_eval(f, x, v, J);
         return;
     }
     case ORC_FIELD_TIME_GRID: {

@@ -27,12 +27,15 @@ enum {
     ORC_FIELD_GRID = 4,      /* trilinear on a uniform Cartesian grid */
     ORC_FIELD_UXB = 5,       /* E = -u x B(x, t), B = *src */
     ORC_FIELD_TIME_GRID = 6, /* LazyTimeInterpolator over gridded snapshots */
+    ORC_FIELD_MESH = 7,      /* rectilinear / cubic / spherical / 2D / 1D grids */
     ORC_FIELD_KAT_FRONT = 100,      /* test_boris.jl:12-21 time_varying_B */
     ORC_FIELD_KAT_SWITCH_ON = 101,  /* test_boris.jl:252 B = t > t_on ? v : 0 */
     ORC_FIELD_KAT_NAN_OUTSIDE = 102 /* test_boundary.jl:10-26 */
 };
 
-enum { ORC_EXTRAP_FILL_NAN = 0, ORC_EXTRAP_CLAMP = 1, ORC_EXTRAP_PERIODIC = 2 };
+enum { ORC_EXTRAP_FILL_NAN = 0, ORC_EXTRAP_CLAMP = 1, ORC_EXTRAP_PERIODIC = 2,
+       ORC_EXTRAP_PERIODIC_INCL = 3 };
+enum { ORC_MESH_CARTESIAN = 0, ORC_MESH_SPHERICAL = 1 };
 enum { ORC_DTYPE_F64 = 0, ORC_DTYPE_F32 = 1 };
 
 typedef struct orc_field {
@@ -55,6 +58,12 @@ typedef struct orc_field {
      * data (3 nx ny nz values each), at strictly increasing times[nsnap] */
     const double* times;
     long nsnap;
+    /* MESH: axes[d] holds the nx / ny / nz node coordinates of axis d;
+     * coord ORC_MESH_*; order 1 or 3; mesh_extrap[d] the boundary per axis */
+    const double* axes[3];
+    int coord, order;
+    int mesh_extrap[3];
+    int pad2_;
 } orc_field;
 
 enum { ORC_ALG_BORIS = 0, ORC_ALG_MULTISTEP = 1 };

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -38,8 +39,10 @@ namespace {
 // ---------------------------------------------------------------------------
 // Precision policies
 // ---------------------------------------------------------------------------
-struct S64 {
+template <bool MESH>
+struct S64T {
     static constexpr int kNS = -1;   // generic: any algorithm, adaptive or not
+    static constexpr bool kMesh = MESH;
     using R = double;
     struct Pos { double x[3]; };
     Pos pos;
@@ -55,9 +58,9 @@ struct S64 {
     __device__ __forceinline__ void set_dt(double dt) { dt_cache = dt; }
     __device__ __forceinline__ void fields(const KParams& p, double t, double E[3], double B[3]) const
     {
-        field_f64(p.B, pos.x, t, B);
+        field_f64<MESH>(p.B, pos.x, t, B);
         if (p.e_is_uxb) cross3(B, p.E.v, E);       // E = -u x B = B x u
-        else field_f64(p.E, pos.x, t, E);
+        else field_f64<MESH>(p.E, pos.x, t, E);
     }
     // fields at the current position, then the velocity update from given
     // fields: update_velocity (boris.jl:38-45) / MultistepUpdater
@@ -96,8 +99,10 @@ struct S64 {
     }
 };
 
-struct S32C {
+template <bool MESH>
+struct S32CT {
     static constexpr int kNS = -1;
+    static constexpr bool kMesh = MESH;
     using R = float;
     using Pos = Pos32;
     Pos pos;
@@ -130,12 +135,12 @@ struct S32C {
     }
     __device__ __forceinline__ void fields(const KParams& p, const Pos& q, double t, float E[3], float B[3]) const
     {
-        field_f32(p, p.B, q, t, B);
+        field_f32<MESH>(p, p.B, q, t, B);
         if (p.e_is_uxb) {
             const float u[3] = {(float)p.E.v[0], (float)p.E.v[1], (float)p.E.v[2]};
             cross3f(B, u, E);
         } else {
-            field_f32(p, p.E, q, t, E);
+            field_f32<MESH>(p, p.E, q, t, E);
         }
     }
     __device__ __forceinline__ void eval(const KParams& p, double t, float E[3], float B[3]) const
@@ -188,6 +193,12 @@ struct S32C {
     }
 };
 
+// generic policies without and with mesh fields
+using S64 = S64T<false>;
+using S64M = S64T<true>;
+using S32C = S32CT<false>;
+using S32CM = S32CT<true>;
+
 template <class S>
 __device__ __forceinline__ bool outside(const KParams& p, const S& s, const typename S::Pos& xn,
                                         const typename S::R vv[3])
@@ -226,13 +237,14 @@ __device__ __forceinline__ bool outside(const KParams& p, const S& s, const type
 // extra record columns (E, B and/or work rates) at (x, v, t), in FP64.  Only
 // the REC instantiations of the generic kernels contain it, so the pushers
 // that save plain records keep their register budget.
+template <bool MESH>
 __device__ __forceinline__ void record_extra(const KParams& p, const double x[3], const double v[3],
                                           double t, double* r)
 {
     double E[3], B[3], J[3][3], dBdt[3];
-    field_jac_f64(p.B, x, t, B, J, dBdt);
+    field_jac_f64<MESH>(p.B, x, t, B, J, dBdt);
     if (p.e_is_uxb) cross3(B, p.E.v, E);                  // E = -u x B = B x u
-    else field_f64(p.E, x, t, E);
+    else field_f64<MESH>(p.E, x, t, E);
     int c = 6;
     if (p.save_fields) {
         r[6] = E[0]; r[7] = E[1]; r[8] = E[2];
@@ -258,7 +270,7 @@ __device__ __forceinline__ void record(const KParams& p, const S& s, long long i
     double* r = p.out_u + (i * p.cap + k) * p.rec_w;
     r[0] = x[0]; r[1] = x[1]; r[2] = x[2];
     r[3] = v[0]; r[4] = v[1]; r[5] = v[2];
-    if constexpr (REC) record_extra(p, x, v, t, r);
+    if constexpr (REC) record_extra<S::kMesh>(p, x, v, t, r);
     p.out_t[i * p.cap + k] = t;
 }
 
@@ -901,7 +913,7 @@ __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const doubl
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (prec == 0) {
-        S64 s;
+        S64M s;
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.pos.x[c] = x[3 * i + c];
         double E[3], B[3];
@@ -909,7 +921,7 @@ __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const doubl
         const double* o = which ? E : B;
         for (int c = 0; c < 3; ++c) out[3 * i + c] = o[c];
     } else {
-        S32C s;
+        S32CM s;
         const double tmp[6] = {x[3 * i], x[3 * i + 1], x[3 * i + 2], 0.0, 0.0, 0.0};
         s.load_from(p, tmp);
         float E[3], B[3];
@@ -932,7 +944,7 @@ __global__ void __launch_bounds__(256) k_record_eval(const KParams p, const doub
         x[c] = xv[6 * i + c];
         v[c] = xv[6 * i + 3 + c];
     }
-    record_extra(p, x, v, t[i], r);
+    record_extra<true>(p, x, v, t[i], r);
     const int w = p.rec_w - 6;
     for (int c = 0; c < w; ++c) out[(size_t)w * i + c] = r[6 + c];
 }
@@ -948,6 +960,7 @@ struct tpamd_field {
     void* cells_data;        // owned cell-replicated copy (grids, optional)
     const tpamd_field* src;  // UxB: the B field it was built from
     size_t bytes;
+    MeshGrid mesh_host;      // Mesh: host copy of the descriptor (device pointers)
 };
 
 struct tpamd_ctx {
@@ -1064,6 +1077,9 @@ int check_desc(tpamd_ctx* ctx, const tpamd_solve_desc* d, int64_t* nt, int64_t* 
 
 bool is_gridded(const DevField& f) { return f.kind == kFieldGrid || f.kind == kFieldTimeGrid; }
 
+// the solve's E or B is a mesh: the generic kernels must carry the mesh evaluator
+bool has_mesh(const KParams& p) { return p.B.kind == kFieldMesh || (!p.e_is_uxb && p.E.kind == kFieldMesh); }
+
 bool same_geometry(const DevField& a, const DevField& b)
 {
     for (int d = 0; d < 3; ++d)
@@ -1172,7 +1188,7 @@ int spec_class(const tpamd_solve_desc* d, const KParams& p, int* emode)
 }
 
 template <class L>
-void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec, const L& l)
+void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec, bool mesh, const L& l)
 {
     int emode = 0;
     const int ns = allow_spec ? spec_class(d, p, &emode) : -1;
@@ -1200,11 +1216,21 @@ void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec,
         return;
     }
     if (d->precision == TPAMD_FP64) {
-        if (p.alg == 0) l.template go<S64, 0>();
-        else l.template go<S64, 1>();
+        if (mesh) {
+            if (p.alg == 0) l.template go<S64M, 0>();
+            else l.template go<S64M, 1>();
+        } else {
+            if (p.alg == 0) l.template go<S64, 0>();
+            else l.template go<S64, 1>();
+        }
     } else {
-        if (p.alg == 0) l.template go<S32C, 0>();
-        else l.template go<S32C, 1>();
+        if (mesh) {
+            if (p.alg == 0) l.template go<S32CM, 0>();
+            else l.template go<S32CM, 1>();
+        } else {
+            if (p.alg == 0) l.template go<S32C, 0>();
+            else l.template go<S32C, 1>();
+        }
     }
 }
 
@@ -1354,11 +1380,15 @@ int fill_params(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field*
     for (int d = 0; d < 3; ++d) { p.box_lo[d] = desc->box_lo[d]; p.box_hi[d] = desc->box_hi[d]; }
     if (desc->bnd_flags & TPAMD_BND_GRID) {
         // open boundary at the faces of B's grid: the node span becomes the box
-        if (!is_gridded(B->d))
-            return fail(ctx, TPAMD_ERR_ARGUMENT, "TPAMD_BND_GRID needs a gridded B field");
+        const bool mesh = B->d.kind == kFieldMesh;
+        if (!is_gridded(B->d) && !(mesh && B->mesh_host.coord == kMeshCartesian))
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "TPAMD_BND_GRID needs a Cartesian gridded B field");
         for (int d = 0; d < 3; ++d) {
-            const double lo = B->d.origin[d];
-            const double hi = B->d.origin[d] + (double)(B->d.n[d] - 1) / B->d.inv_h[d];
+            const MeshAxis& ma = B->mesh_host.ax[d];
+            const double inf = std::numeric_limits<double>::infinity();
+            const double lo = mesh ? (ma.n == 1 ? -inf : ma.lo) : B->d.origin[d];
+            const double hi = mesh ? (ma.n == 1 ? inf : ma.hi)
+                                   : B->d.origin[d] + (double)(B->d.n[d] - 1) / B->d.inv_h[d];
             if (!(desc->bnd_flags & TPAMD_BND_BOX)) { p.box_lo[d] = lo; p.box_hi[d] = hi; }
             else { p.box_lo[d] = std::fmax(p.box_lo[d], lo); p.box_hi[d] = std::fmin(p.box_hi[d], hi); }
         }
@@ -1392,7 +1422,7 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     if (N > 0) {
         const dim3 block(bs), grid((unsigned)((N + bs - 1) / bs));
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-        select_policy(desc, p, true, SoloLaunch{p, desc->adaptive, grid, block, st});
+        select_policy(desc, p, true, has_mesh(p), SoloLaunch{p, desc->adaptive, grid, block, st});
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     }
@@ -1432,6 +1462,7 @@ struct tpamd_batch {
     unsigned* d_counter;
     unsigned long long* d_steps;
     bool allow_spec;
+    bool any_mesh;
     int block_size;
     int ppt;
     int n_cu;
@@ -1480,7 +1511,7 @@ int batch_enqueue(tpamd_batch* b, hipStream_t st)
     const bool rec = TPAMD_RECORD_WIDTH(&b->desc) > 6;
     BatchLaunch q{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
                   dim3(1), block, st, &occ, rec};
-    select_policy(&b->desc, b->table[0], b->allow_spec, q);
+    select_policy(&b->desc, b->table[0], b->allow_spec, b->any_mesh, q);
     occ = std::max(occ, 1);
     const long long waves_needed = b->nchunks;
     // The occupancy query counts 256 VGPRs per lane, half of what gfx950
@@ -1494,7 +1525,7 @@ int batch_enqueue(tpamd_batch* b, hipStream_t st)
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counter, b->d_steps);
     BatchLaunch l{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
                   dim3((unsigned)blocks), block, st, nullptr, rec};
-    select_policy(&b->desc, b->table[0], b->allow_spec, l);
+    select_policy(&b->desc, b->table[0], b->allow_spec, b->any_mesh, l);
     HIP_TRY(ctx, hipGetLastError());
     return TPAMD_OK;
 }
@@ -1900,6 +1931,144 @@ int32_t tpamd_field_time_grid(tpamd_ctx* ctx, const void* const* snapshots, int3
     return TPAMD_OK;
 }
 
+int32_t tpamd_field_mesh(tpamd_ctx* ctx, const void* data, int32_t dtype, const int64_t n[3],
+                         const double* const axes[3], int32_t coord, int32_t order,
+                         const int32_t extrap[3], int32_t storage_dtype, tpamd_field** out)
+{
+    if (!ctx || !out || !data || !n || !axes || !extrap)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
+    if (coord != kMeshCartesian && coord != kMeshSpherical)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "unknown mesh coordinates %d", coord);
+    if (order != 1 && order != 3)
+        return fail(ctx, TPAMD_ERR_UNSUPPORTED, "interpolation order %d is not provided (1 or 3)", order);
+    if ((dtype != TPAMD_F64 && dtype != TPAMD_F32) ||
+        (storage_dtype != TPAMD_F64 && storage_dtype != TPAMD_F32))
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "dtype must be TPAMD_F64 or TPAMD_F32");
+    MeshGrid g;
+    std::memset(&g, 0, sizeof g);
+    std::vector<std::vector<int>> buckets(3);
+    int active = 0;
+    for (int d = 0; d < 3; ++d) {
+        MeshAxis& a = g.ax[d];
+        if (n[d] < 1 || n[d] > (1 << 30)) return fail(ctx, TPAMD_ERR_ARGUMENT, "mesh axis %d: bad size", d);
+        if (!axes[d]) return fail(ctx, TPAMD_ERR_ARGUMENT, "mesh axis %d: null coordinates", d);
+        if (extrap[d] < 0 || extrap[d] > kExtrapPeriodicIncl)
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "mesh axis %d: unknown boundary %d", d, extrap[d]);
+        const double* c = axes[d];
+        a.n = (int)n[d];
+        a.bc = extrap[d];
+        a.lo = c[0];
+        a.hi = c[a.n - 1];
+        for (int i = 0; i < a.n; ++i) {
+            if (!std::isfinite(c[i])) return fail(ctx, TPAMD_ERR_ARGUMENT, "mesh axis %d: coordinates must be finite", d);
+            if (i > 0 && !(c[i] > c[i - 1]))
+                return fail(ctx, TPAMD_ERR_ARGUMENT, "Grid vectors must be sorted (mesh axis %d).", d);
+        }
+        if (a.n == 1) continue;
+        ++active;
+        const double h = (a.hi - a.lo) / (double)(a.n - 1);
+        bool uni = true;
+        double hmin = a.hi - a.lo;
+        for (int i = 1; i < a.n; ++i) {
+            hmin = std::fmin(hmin, c[i] - c[i - 1]);
+            if (std::fabs(c[i] - (a.lo + (double)i * h)) > 1e-9 * (a.hi - a.lo)) uni = false;
+        }
+        a.uniform = uni ? 1 : 0;
+        a.inv_h = uni ? 1.0 / h : 0.0;
+        if (a.bc == kExtrapPeriodic) {
+            if (!uni)
+                return fail(ctx, TPAMD_ERR_UNSUPPORTED, "mesh axis %d: exclusive periodic wrap needs a uniform axis", d);
+            a.period = (double)a.n * h;
+        } else if (a.bc == kExtrapPeriodicIncl) {
+            a.period = a.hi - a.lo;
+        }
+        if (order == 3 && !uni)
+            return fail(ctx, TPAMD_ERR_UNSUPPORTED, "order 3 needs uniform axes (mesh axis %d is not)", d);
+        if (!uni) {
+            // bucket b covers [lo + b bw, lo + (b+1) bw): its entry is the cell
+            // holding the bucket's left end; the device walks from there
+            const double span = a.hi - a.lo;
+            double want = std::ceil(span / hmin);
+            const double cap = 64.0 * (double)(a.n - 1);
+            want = want < (double)(a.n - 1) ? (double)(a.n - 1) : want;
+            want = want > cap ? cap : want;
+            a.nb = (int)want;
+            a.inv_bw = (double)a.nb / span;
+            std::vector<int>& bk = buckets[d];
+            bk.resize(a.nb);
+            int i = 0;
+            for (int b = 0; b < a.nb; ++b) {
+                const double xb = a.lo + (double)b * (span / (double)a.nb);
+                while (i < a.n - 2 && c[i + 1] <= xb) ++i;
+                bk[b] = i;
+            }
+        }
+    }
+    if (active == 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "a mesh needs an axis with at least 2 nodes");
+    if (coord == kMeshSpherical) {
+        // build_interpolator(StructuredGrid, ...) (interpolation.jl:260-284)
+        if (g.ax[0].n < 2 || g.ax[1].n < 2 || g.ax[2].n < 2)
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "a spherical mesh needs 2 or more nodes per axis");
+        if (!(g.ax[0].lo >= 0.0)) return fail(ctx, TPAMD_ERR_ARGUMENT, "r must be non-negative.");
+        const double pi = 3.141592653589793;   // the double nearest pi
+        if (!(g.ax[1].lo >= 0.0 && g.ax[1].hi <= pi))
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "θ must be within [0, π].");
+        if (!(g.ax[2].lo >= 0.0 && g.ax[2].hi <= 2.0 * pi))
+            return fail(ctx, TPAMD_ERR_ARGUMENT, "ϕ must be within [0, 2π].");
+    }
+    g.coord = coord;
+    g.order = order;
+    g.dtype = storage_dtype == TPAMD_F64 ? 0 : 1;
+    g.sy = n[0];
+    g.sz = n[0] * n[1];
+    const size_t nn = (size_t)n[0] * (size_t)n[1] * (size_t)n[2] * 3;
+    const size_t es = storage_dtype == TPAMD_F64 ? 8 : 4;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    size_t off = up(sizeof(MeshGrid));
+    size_t off_c[3], off_b[3];
+    for (int d = 0; d < 3; ++d) {
+        off_c[d] = off;
+        off += up((size_t)n[d] * 8);
+        off_b[d] = off;
+        off += up(buckets[d].size() * sizeof(int));
+    }
+    const size_t off_data = off;
+    const size_t total = off_data + nn * es;
+    char* dev = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMalloc((void**)&dev, total));
+    hipError_t e = hipSuccess;
+    for (int d = 0; d < 3 && e == hipSuccess; ++d) {
+        g.ax[d].c = (const double*)(dev + off_c[d]);
+        g.ax[d].bucket = buckets[d].empty() ? nullptr : (const int*)(dev + off_b[d]);
+        e = hipMemcpy(dev + off_c[d], axes[d], (size_t)n[d] * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !buckets[d].empty())
+            e = hipMemcpy(dev + off_b[d], buckets[d].data(), buckets[d].size() * sizeof(int),
+                          hipMemcpyHostToDevice);
+    }
+    g.data = dev + off_data;
+    if (e == hipSuccess) e = upload_nodes(dev + off_data, data, dtype, nn, storage_dtype);
+    if (e == hipSuccess) e = hipMemcpy(dev, &g, sizeof g, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return fail(ctx, TPAMD_ERR_DEVICE, "mesh upload failed: %s", hipGetErrorString(e));
+    }
+    int rc = new_field(ctx, out, kFieldMesh);
+    if (rc) {
+        (void)hipFree(dev);
+        return rc;
+    }
+    DevField& f = (*out)->d;
+    f.mesh = (const MeshGrid*)dev;
+    f.data = dev + off_data;
+    f.dtype = g.dtype;
+    for (int d = 0; d < 3; ++d) f.n[d] = (int)n[d];
+    (*out)->dev_data = dev;
+    (*out)->bytes = total;
+    (*out)->mesh_host = g;
+    return TPAMD_OK;
+}
+
 int32_t tpamd_field_uxb(tpamd_ctx* ctx, const double u[3], const tpamd_field* B, tpamd_field** out)
 {
     if (!u || !B) return fail(ctx, TPAMD_ERR_ARGUMENT, "null argument");
@@ -2251,6 +2420,8 @@ int32_t tpamd_batch_create(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t
     int em0 = 0;
     const int ns0 = spec_class(desc, b->table[0], &em0);
     b->allow_spec = ns0 >= 0;
+    b->any_mesh = false;
+    for (int i = 0; i < n_inst; ++i) b->any_mesh = b->any_mesh || has_mesh(b->table[i]);
     for (int i = 1; i < n_inst && b->allow_spec; ++i) {
         int em = 0;
         const int ns = spec_class(desc, b->table[i], &em);

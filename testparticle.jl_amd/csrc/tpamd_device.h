@@ -25,6 +25,8 @@ constexpr int kRationalNMax = 12;
 // adaptive solves without a step cap stop after this many steps per particle
 constexpr long long kAdaptiveHardCap = 1LL << 31;
 
+struct MeshGrid;
+
 struct DevField {
     int kind;        // tpamd_field_kind
     int extrap;      // tpamd_extrap (grids)
@@ -56,6 +58,8 @@ struct DevField {
     long long snap_bytes;
     int nsnap;
     int pad3;
+    // kind Mesh: rectilinear / cubic / spherical grid descriptor (device)
+    const MeshGrid* mesh;
 };
 
 struct Poly {
@@ -111,8 +115,11 @@ struct KParams {
 };
 
 enum { kFieldZero = 0, kFieldUniform = 1, kFieldAffine = 2, kFieldDipole = 3, kFieldGrid = 4,
-       kFieldUxB = 5, kFieldTimeGrid = 6 };
+       kFieldUxB = 5, kFieldTimeGrid = 6, kFieldMesh = 7 };
 enum { kExtrapFillNaN = 0, kExtrapClamp = 1, kExtrapPeriodic = 2 };
+}  // namespace tpamd
+#include "tpamd_mesh.h"
+namespace tpamd {
 enum { kBndNaN = 1, kBndSphereIn = 2, kBndSphereOut = 4, kBndBox = 8 };
 enum { kRetSuccess = 0, kRetTerminated = 1, kRetMaxIters = 2 };
 
@@ -347,8 +354,18 @@ __device__ __forceinline__ const void* snapshot(const DevField& f, int k)
     return (const char*)f.data + (size_t)k * (size_t)f.snap_bytes;
 }
 
+// MESH instantiations also evaluate mesh fields (out of line); the others
+// leave them out, so kernels that never see a mesh do not carry its call
+template <bool MESH = false>
 __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], double t, double o[3])
 {
+    if constexpr (MESH) {
+        if (f.kind == kFieldMesh) {
+            const MeshVal r = mesh_value(f.mesh, x[0], x[1], x[2]);
+            o[0] = r.v[0]; o[1] = r.v[1]; o[2] = r.v[2];
+            return;
+        }
+    }
     switch (f.kind) {
     case kFieldUniform:
         o[0] = f.v[0]; o[1] = f.v[1]; o[2] = f.v[2];
@@ -390,7 +407,7 @@ __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], 
         for (int c = 0; c < 3; ++c) o[c] = (1.0 - w) * a[c] + w * b[c];   // interpolation.jl:377
         return;
     }
-    default:  // zero
+    default:  // zero, and meshes outside MESH instantiations
         o[0] = o[1] = o[2] = 0.0;
         return;
     }
@@ -475,10 +492,11 @@ __device__ __forceinline__ void grid_jac_f64(const DevField& f, const void* data
 // differentiates the field callables with ForwardDiff; here each field kind
 // has its derivative in closed form.  B itself is field_f64's value.
 // ---------------------------------------------------------------------------
+template <bool MESH = false>
 __device__ __forceinline__ void field_jac_f64(const DevField& f, const double x[3], double t,
                                               double B[3], double J[3][3], double dBdt[3])
 {
-    field_f64(f, x, t, B);
+    field_f64<MESH>(f, x, t, B);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         dBdt[i] = 0.0;
@@ -538,6 +556,15 @@ __device__ __forceinline__ void field_jac_f64(const DevField& f, const double x[
         for (int i = 0; i < 3; ++i) dBdt[i] = (b[i] - a[i]) / dtk;
         return;
     }
+    case kFieldMesh:
+        if constexpr (MESH) {
+            const MeshValJac r = mesh_value_jac(f.mesh, x[0], x[1], x[2]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) J[a][b] = r.J[3 * a + b];
+        }
+        return;
     default:  // zero, uniform
         return;
     }
@@ -793,9 +820,20 @@ __device__ __forceinline__ void pos_abs(const KParams& p, const Pos32& x, double
                ((double)x.hi[d] + (double)x.lo[d]);
 }
 
+template <bool MESH = false>
 __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, const Pos32& x,
                                           double t, float o[3])
 {
+    if constexpr (MESH) {
+        if (f.kind == kFieldMesh) {
+            // meshes are evaluated in FP64 at the absolute position
+            double xa[3];
+            pos_abs(p, x, xa);
+            const MeshVal r = mesh_value(f.mesh, xa[0], xa[1], xa[2]);
+            o[0] = (float)r.v[0]; o[1] = (float)r.v[1]; o[2] = (float)r.v[2];
+            return;
+        }
+    }
     switch (f.kind) {
     case kFieldUniform:
         o[0] = (float)f.v[0]; o[1] = (float)f.v[1]; o[2] = (float)f.v[2];

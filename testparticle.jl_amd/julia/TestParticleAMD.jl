@@ -112,6 +112,21 @@ struct UxBDeviceField <: DeviceField
     B::DeviceField
 end
 """
+Gridded field on a general mesh: `build_interpolator(gridtype, A, g1, g2, g3, order, bc)`
+for RectilinearGrid / non-uniform CartesianGrid axes, order 3 (uniform axes),
+StructuredGrid (`coord = 1`, axes r, θ, ϕ) and 2D / 1D fields (an axis of
+one node is ignored). `extrap` per axis: 0 fill NaN, 1 clamp, 2 exclusive
+periodic, 3 inclusive periodic.
+"""
+struct MeshDeviceField{T} <: DeviceField
+    A::Array{T, 4}
+    axes::NTuple{3, Vector{Float64}}
+    coord::Int32
+    order::Int32
+    extrap::NTuple{3, Int32}
+    storage::Int32
+end
+"""
 Time-dependent gridded field, the device form of
 `LazyTimeInterpolator(times, loader)` (src/utility/interpolation.jl:337-430):
 `loader(i)` returns the `GridDeviceField` of snapshot i on one shared grid.
@@ -143,6 +158,19 @@ function handle(ctx, f::GridDeviceField{T}, cache) where {T}
                  Int32, Int32, Ptr{Ptr{Cvoid}}),
                 ctx, f.A, T === Float64 ? 0 : 1, size(f.A, 2), size(f.A, 3), size(f.A, 4),
                 origin, spacing, f.extrap, f.storage, h)
+        end
+    end
+end
+function handle(ctx, f::MeshDeviceField{T}, cache) where {T}
+    get!(cache, f) do
+        _mk(ctx) do h
+            n = Int64[size(f.A, 2), size(f.A, 3), size(f.A, 4)]
+            axp = [pointer(a) for a in f.axes]
+            ex = Int32[f.extrap...]
+            GC.@preserve f ccall((:tpamd_field_mesh, libtpamd), Int32,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Ptr{Float64}}, Int32, Int32,
+                 Ptr{Int32}, Int32, Ptr{Ptr{Cvoid}}),
+                ctx, f.A, T === Float64 ? 0 : 1, n, axp, f.coord, f.order, ex, f.storage, h)
         end
     end
 end

@@ -17,7 +17,7 @@ from .algorithms import (AdaptiveBoris, AdaptiveMultistepBoris, Boris, EnsembleD
 from .constants import (BMoment_Earth, Electron, Ion, Proton, Re, Species, SpeciesDict, c, eV,
                         get_gyrofrequency, get_gyroperiod, kB, me, mi, qe, qi)
 from .fields import (AffineField, ClampExtrap, DipoleField, FieldInterpolator, FillExtrap,
-                     GridField, LazyTimeInterpolator, PeriodicBC, UniformField, UxBField,
+                     GridField, LazyTimeInterpolator, MeshField, PeriodicBC, UniformField, UxBField,
                      WrapExtrap, ZeroField, build_interpolator, prepare)
 from .problem import (Boundary, EnsembleContext, InsideSphere, OutsideBox, OutsideGrid,
                       OutsideNaN, OutsideSphere, TerminateOutside, TraceProblem, remake)

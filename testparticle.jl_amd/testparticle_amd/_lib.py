@@ -18,8 +18,10 @@ ABI_VERSION = 1
 OK, ERR_ARGUMENT, ERR_DT_TOO_SMALL, ERR_MAXITERS, ERR_ORDER = 0, 1, 2, 3, 4
 ERR_DEVICE, ERR_OOM, ERR_UNSUPPORTED, ERR_CAPACITY = 5, 6, 7, 8
 # enums
-FIELD_ZERO, FIELD_UNIFORM, FIELD_AFFINE, FIELD_DIPOLE, FIELD_GRID, FIELD_UXB, FIELD_TIME_GRID = range(7)
-EXTRAP_FILL_NAN, EXTRAP_CLAMP, EXTRAP_PERIODIC = 0, 1, 2
+FIELD_ZERO, FIELD_UNIFORM, FIELD_AFFINE, FIELD_DIPOLE, FIELD_GRID, FIELD_UXB = range(6)
+FIELD_TIME_GRID, FIELD_MESH = 6, 7
+EXTRAP_FILL_NAN, EXTRAP_CLAMP, EXTRAP_PERIODIC, EXTRAP_PERIODIC_INCLUSIVE = 0, 1, 2, 3
+MESH_CARTESIAN, MESH_SPHERICAL = 0, 1
 F64, F32 = 0, 1
 ALG_BORIS, ALG_MULTISTEP = 0, 1
 FORM_REFERENCE, FORM_RATIONAL = 0, 1
@@ -75,7 +77,8 @@ class Stats(C.Structure):
 EXPORTS = [
     "tpamd_abi_version", "tpamd_init", "tpamd_finalize", "tpamd_last_error",
     "tpamd_field_zero", "tpamd_field_uniform", "tpamd_field_affine", "tpamd_field_dipole",
-    "tpamd_field_grid", "tpamd_field_time_grid", "tpamd_field_grid_cells", "tpamd_field_uxb",
+    "tpamd_field_grid", "tpamd_field_time_grid", "tpamd_field_mesh", "tpamd_field_grid_cells",
+    "tpamd_field_uxb",
     "tpamd_field_eval",
     "tpamd_field_free", "tpamd_record_eval",
     "tpamd_plan", "tpamd_solve", "tpamd_solve_device", "tpamd_adjoint",
@@ -130,6 +133,8 @@ def load_library(path: str = LIB_PATH):
                                        C.c_int32, pp]
         L.tpamd_field_time_grid.argtypes = [vp, pp, C.c_int32, i64, i64, i64, C.c_int32, d3, d3,
                                             d3, C.c_int32, C.c_int32, pp]
+        L.tpamd_field_mesh.argtypes = [vp, vp, C.c_int32, C.POINTER(i64), C.POINTER(d3), C.c_int32,
+                                       C.c_int32, C.POINTER(C.c_int32), C.c_int32, pp]
         L.tpamd_field_uxb.argtypes = [vp, d3, vp, pp]
         L.tpamd_field_grid_cells.argtypes = [vp, vp, C.c_int32]
         L.tpamd_adjoint.argtypes = [vp, C.POINTER(SolveDesc), vp, d3, i64, d3, C.c_int32, d3,

@@ -29,6 +29,10 @@ def oracle_field(f, B_oracle=None):
         return O.grid(data, f.origin, f.spacing, f.extrap)
     if isinstance(f, TP.UxBField):
         return O.uxb(f.u, B_oracle)
+    if isinstance(f, TP.MeshField):
+        data = f.data.astype(np.float32 if f.storage == "f32" else np.float64, copy=False)
+        coord = O.MESH_SPHERICAL if f.coord == "spherical" else O.MESH_CARTESIAN
+        return O.mesh(data, f.axes, coord, f.order, f.extrap)
     if isinstance(f, TP.LazyTimeInterpolator):
         dt = np.float32 if f.storage == "f32" else np.float64
         snaps = np.stack([g.data.astype(dt, copy=False) for _, g in f.snapshots()])
@@ -54,8 +58,12 @@ def run_oracle(prob, alg, *, dt=None, savestepinterval=1, isoutside=None, save_s
             # the grid's node span, computed as the library does: origin + (n-1) / (1/h)
             g = prob.p[3]
             for d in range(3):
-                glo = float(g.origin[d])
-                ghi = glo + float(g.shape[d] - 1) / (1.0 / float(g.spacing[d]))
+                if isinstance(g, TP.MeshField):      # a mesh's node box; one node: unbounded
+                    a = g.axes[d]
+                    glo, ghi = (-np.inf, np.inf) if len(a) == 1 else (float(a[0]), float(a[-1]))
+                else:
+                    glo = float(g.origin[d])
+                    ghi = glo + float(g.shape[d] - 1) / (1.0 / float(g.spacing[d]))
                 if flags & _lib.BND_BOX:
                     lo[d], hi[d] = max(lo[d], glo), min(hi[d], ghi)
                 else:

@@ -118,8 +118,22 @@ def test_grid_layout_matches_reference_indexing():
         TP.GridField(A[:2], x, y, z)
     with pytest.raises(NotImplementedError):
         TP.GridField(A, np.array([0, 1, 3, 4.0]), y, z)
-    with pytest.raises(NotImplementedError):
-        TP.build_interpolator(A, x, y, z, 3)
+    # order 3, non-uniform axes, 2D / 1D and spherical grids route to a mesh
+    assert isinstance(TP.build_interpolator(A, x, y, z, 3), TP.MeshField)
+    assert isinstance(TP.build_interpolator(A, np.array([0, 1, 3, 4.0]), y, z), TP.MeshField)
+    assert isinstance(TP.build_interpolator(A, x, y, z), TP.GridField)
+    m2 = TP.build_interpolator(A[..., 0], x, y)
+    assert m2.shape == (4, 5, 1) and [len(a) for a in m2.axes] == [4, 5, 1]
+    m1 = TP.build_interpolator(A[:, 0, 0, :], z, dir=3)
+    assert m1.shape == (1, 1, 6)
+    with pytest.raises(NotImplementedError):           # constant (order 0) interpolation
+        TP.build_interpolator(A, x, y, z, 0)
+    with pytest.raises(TP.ArgumentError):              # test_utility.jl:236-238
+        TP.build_interpolator(A, x, y, z, 2)
+    with pytest.raises(TP.ArgumentError):              # interpolation.jl:250-252
+        TP.build_interpolator("RectilinearGrid", A, x, y, z, 3)
+    with pytest.raises(TP.ArgumentError):              # interpolation.jl:275-277
+        TP.build_interpolator("StructuredGrid", A, x, y, z)
 
 
 def test_initial_states_prob_func():
