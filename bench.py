@@ -248,8 +248,8 @@ def bench_dipole(args, torch, TP, W, DeviceEnsemble):
                      "terminated": int(np.sum(ret == 1)),
                      "roofline": {"bound": "valu", "achieved": flops / 1e12, "peak": peak / 1e12,
                                   "unit": "TFLOP/s", "frac": flops / peak,
-                                  "kernel": "k_fixed<S64, Boris>" if prec == "fp64"
-                                  else "k_fixed<S32C, Boris>"}}
+                                  "kernel": "k_fixed<S64A, Boris>" if prec == "fp64"
+                                  else "k_fixed<S32CA, Boris>"}}
         log(f"[bench] dipole {prec}: {rate:.3e} particle-steps/s ({flops / 1e12:.1f} TFLOP/s)")
         del ens
     u64, r64 = fin["fp64"]

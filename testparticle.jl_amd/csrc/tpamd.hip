@@ -1,13 +1,7 @@
-// tpamd.hip -- kernels and C ABI of the MI355X ensemble Boris pusher.
-//
-// One thread per particle; the whole integration (all nt steps, every save)
-// runs inside one launch, with the particle state in registers.  This
-// replaces the reference's accelerator path, which launches one kernel per
-// step, synchronises, and copies the full 6 x N state to the host for every
-// saved record (src/boris/boris_kernel.jl:130-236).
-#include "tpamd_device.h"
-#include "tpamd_grid.h"
-#include "tpamd_adjoint.h"
+// tpamd.hip -- host side and C ABI of the MI355X ensemble Boris pusher: fields,
+// plans, solves, batches and the adjoint's entry; the kernels live in the
+// family units (tpamd_gen64/32[m], tpamd_spec64/32, tpamd_adj).
+#include "tpamd_pusher.h"
 #include "tpamd.h"
 
 #include <hip/hip_runtime.h>
@@ -24,847 +18,7 @@
 #include <type_traits>
 #include <vector>
 
-using namespace tpamd;
-
-namespace tpamd {
-// tpamd_sort.hip: radix sort of (key, record) pairs on key bits [begin, end)
-template <typename R>
-hipError_t adj_sort_records(void* d_temp, size_t& temp_bytes, const unsigned* keys_in,
-                            unsigned* keys_out, const AdjRec<R>* vals_in, AdjRec<R>* vals_out,
-                            long long n, int begin_bit, int end_bit, hipStream_t stream);
-}  // namespace tpamd
-
 namespace {
-
-// ---------------------------------------------------------------------------
-// Precision policies
-// ---------------------------------------------------------------------------
-template <bool MESH>
-struct S64T {
-    static constexpr int kNS = -1;   // generic: any algorithm, adaptive or not
-    static constexpr bool kMesh = MESH;
-    using R = double;
-    struct Pos { double x[3]; };
-    Pos pos;
-    double v[3];
-    double dt_cache;
-
-    __device__ __forceinline__ void load(const KParams& p, long long i)
-    {
-        const double* s = p.xv0 + 6 * i;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { pos.x[c] = s[c]; v[c] = s[3 + c]; }
-    }
-    __device__ __forceinline__ void set_dt(double dt) { dt_cache = dt; }
-    __device__ __forceinline__ void fields(const KParams& p, double t, double E[3], double B[3]) const
-    {
-        field_f64<MESH>(p.B, pos.x, t, B);
-        if (p.e_is_uxb) cross3(B, p.E.v, E);       // E = -u x B = B x u
-        else field_f64<MESH>(p.E, pos.x, t, E);
-    }
-    // fields at the current position, then the velocity update from given
-    // fields: update_velocity (boris.jl:38-45) / MultistepUpdater
-    // (multistep_boris.jl:7-9) is U = eval + Uf
-    __device__ __forceinline__ void eval(const KParams& p, double t, double E[3], double B[3]) const
-    {
-        fields(p, t, E, B);
-    }
-    template <int ALG>
-    __device__ __forceinline__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
-                       double h, double out[3]) const
-    {
-        if (ALG == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
-        else multistep_f64(p, vin, E, B, h, out);
-    }
-    template <int ALG>
-    __device__ __forceinline__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
-    {
-        double E[3], B[3];
-        eval(p, t, E, B);
-        Uf<ALG>(p, E, B, vin, h, out);
-    }
-    __device__ __forceinline__ double bmag_of(const double B[3]) const { return sqrt(dot3(B, B)); }
-    __device__ __forceinline__ void step(const double vv[3], double dt, Pos& xn) const
-    {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) xn.x[c] = pos.x[c] + vv[c] * dt;
-    }
-    __device__ __forceinline__ void abs_pos(const KParams&, const Pos& q, double o[3]) const
-    {
-        o[0] = q.x[0]; o[1] = q.x[1]; o[2] = q.x[2];
-    }
-    __device__ __forceinline__ void vel_d(const double vv[3], double o[3]) const
-    {
-        o[0] = vv[0]; o[1] = vv[1]; o[2] = vv[2];
-    }
-};
-
-template <bool MESH>
-struct S32CT {
-    static constexpr int kNS = -1;
-    static constexpr bool kMesh = MESH;
-    using R = float;
-    using Pos = Pos32;
-    Pos pos;
-    float v[3];
-    float dt_f, dt_hi, dt_lo;
-
-    __device__ __forceinline__ void load(const KParams& p, long long i) { load_from(p, p.xv0 + 6 * i); }
-    __device__ __forceinline__ void load_from(const KParams& p, const double* s)
-    {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double r = s[c] - p.frame_origin[c];
-            int w = 0;
-            if (p.frame_period[c] > 0.0) {
-                const double q = floor(r / p.frame_period[c]);
-                w = (int)q;
-                r = r - q * p.frame_period[c];
-            }
-            pos.hi[c] = (float)r;
-            pos.lo[c] = (float)(r - (double)pos.hi[c]);
-            pos.wrap[c] = w;
-            v[c] = (float)s[3 + c];
-        }
-    }
-    __device__ __forceinline__ void set_dt(double dt)
-    {
-        dt_hi = (float)dt;
-        dt_lo = (float)(dt - (double)dt_hi);
-        dt_f = dt_hi;
-    }
-    __device__ __forceinline__ void fields(const KParams& p, const Pos& q, double t, float E[3], float B[3]) const
-    {
-        field_f32<MESH>(p, p.B, q, t, B);
-        if (p.e_is_uxb) {
-            const float u[3] = {(float)p.E.v[0], (float)p.E.v[1], (float)p.E.v[2]};
-            cross3f(B, u, E);
-        } else {
-            field_f32<MESH>(p, p.E, q, t, E);
-        }
-    }
-    __device__ __forceinline__ void eval(const KParams& p, double t, float E[3], float B[3]) const
-    {
-        fields(p, pos, t, E, B);
-    }
-    template <int ALG>
-    __device__ __forceinline__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
-                       double h, float out[3]) const
-    {
-        if (ALG == 0) boris_f32(vin, E, B, (float)((p.q2m * 0.5) * h), out);
-        else multistep_f32(p, vin, E, B, (float)h, out);
-    }
-    template <int ALG>
-    __device__ __forceinline__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
-    {
-        float E[3], B[3];
-        eval(p, t, E, B);
-        Uf<ALG>(p, E, B, vin, h, out);
-    }
-    __device__ __forceinline__ double bmag_of(const float B[3]) const { return (double)sqrtf(dot3f(B, B)); }
-    __device__ __forceinline__ void step_impl(const KParams& p, const float vv[3], Pos& xn) const
-    {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float d = fmaf(vv[c], dt_hi, vv[c] * dt_lo);
-            float s, e;
-            two_sum(pos.hi[c], d + pos.lo[c], s, e);
-            int w = pos.wrap[c];
-            if (p.frame_periodic) {
-                const float P = (float)p.frame_period[c];
-                if (P > 0.0f && (s >= P || s < 0.0f)) {
-                    const float Plo = (float)(p.frame_period[c] - (double)P);
-                    const float sgn = s >= P ? 1.0f : -1.0f;
-                    float s2, e2;
-                    two_sum(s, -sgn * P, s2, e2);
-                    two_sum(s2, fmaf(-sgn, Plo, e + e2), s, e);
-                    w += (int)sgn;
-                }
-            }
-            xn.hi[c] = s;
-            xn.lo[c] = e;
-            xn.wrap[c] = w;
-        }
-    }
-    __device__ __forceinline__ void abs_pos(const KParams& p, const Pos& q, double o[3]) const { pos_abs(p, q, o); }
-    __device__ __forceinline__ void vel_d(const float vv[3], double o[3]) const
-    {
-        o[0] = (double)vv[0]; o[1] = (double)vv[1]; o[2] = (double)vv[2];
-    }
-};
-
-// generic policies without and with mesh fields
-using S64 = S64T<false>;
-using S64M = S64T<true>;
-using S32C = S32CT<false>;
-using S32CM = S32CT<true>;
-
-template <class S>
-__device__ __forceinline__ bool outside(const KParams& p, const S& s, const typename S::Pos& xn,
-                                        const typename S::R vv[3])
-{
-    const int f = p.bnd_flags;
-    if (!f) return false;
-    double x[3], v[3];
-    s.abs_pos(p, xn, x);
-    s.vel_d(vv, v);
-    if (f & kBndNaN) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if (x[c] != x[c] || v[c] != v[c]) return true;
-    }
-    if (f & (kBndSphereIn | kBndSphereOut)) {
-        // r = sqrt(|x|^2) compared with the radius; away from the sphere the
-        // squared norm alone decides it, near it the exact test runs
-        const double d = dot3(x, x);
-        if (f & kBndSphereIn) {
-            if (d < p.rin2_lo) return true;
-            if (d <= p.rin2_hi && sqrt(d) < p.r_in) return true;
-        }
-        if (f & kBndSphereOut) {
-            if (d > p.rout2_hi) return true;
-            if (d >= p.rout2_lo && sqrt(d) > p.r_out) return true;
-        }
-    }
-    if (f & kBndBox) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if (x[c] < p.box_lo[c] || x[c] > p.box_hi[c]) return true;
-    }
-    return false;
-}
-
-// extra record columns (E, B and/or work rates) at (x, v, t), in FP64.  Only
-// the REC instantiations of the generic kernels contain it, so the pushers
-// that save plain records keep their register budget.
-template <bool MESH>
-__device__ __forceinline__ void record_extra(const KParams& p, const double x[3], const double v[3],
-                                          double t, double* r)
-{
-    double E[3], B[3], J[3][3], dBdt[3];
-    field_jac_f64<MESH>(p.B, x, t, B, J, dBdt);
-    if (p.e_is_uxb) cross3(B, p.E.v, E);                  // E = -u x B = B x u
-    else field_f64<MESH>(p.E, x, t, E);
-    int c = 6;
-    if (p.save_fields) {
-        r[6] = E[0]; r[7] = E[1]; r[8] = E[2];
-        r[9] = B[0]; r[10] = B[1]; r[11] = B[2];
-        c = 12;
-    }
-    if (p.save_work) {
-        double w[4];
-        work_rates_f64(p.q2m, p.mass, v, E, B, J, dBdt, w);
-        r[c] = w[0]; r[c + 1] = w[1]; r[c + 2] = w[2]; r[c + 3] = w[3];
-    }
-}
-
-// one record (_prepare_saved_data, boris.jl:405-440): x, v, then E and B,
-// then the work rates, as the descriptor asks (REC kernels only)
-template <bool REC, class S>
-__device__ __forceinline__ void record(const KParams& p, const S& s, long long i, long long k,
-                                       const typename S::R vv[3], double t)
-{
-    double x[3], v[3];
-    s.abs_pos(p, s.pos, x);
-    s.vel_d(vv, v);
-    double* r = p.out_u + (i * p.cap + k) * p.rec_w;
-    r[0] = x[0]; r[1] = x[1]; r[2] = x[2];
-    r[3] = v[0]; r[4] = v[1]; r[5] = v[2];
-    if constexpr (REC) record_extra<S::kMesh>(p, x, v, t, r);
-    p.out_t[i * p.cap + k] = t;
-}
-
-template <class S>
-__device__ __forceinline__ void do_step(const KParams& p, const S& s, const typename S::R vv[3],
-                                        double dt, typename S::Pos& xn)
-{
-    if constexpr (sizeof(typename S::R) == 8) s.step(vv, dt, xn);
-    else s.step_impl(p, vv, xn);
-}
-
-// _boris_single + _boris_loop!, src/boris/boris.jl:445-557.  The save test
-// `(it - 1) % K == 0` is carried as a countdown to the next save step.
-template <class S, int ALG, bool REC>
-__device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
-{
-    using R = typename S::R;
-    S s;
-    s.load(p, i);
-    const double dt = p.dt_pp ? p.dt_pp[i] : p.dt;
-    const double t0 = p.t0;
-    const double t1 = p.dt_pp ? t0 + (double)p.nt * dt : p.t1;
-    s.set_dt(dt);
-    long long iout = 0;
-    double tlast = 0.0;
-    if (p.save_start) {                                       // :507-511
-        record<REC>(p, s, i, iout, s.v, t0);
-        tlast = t0;
-        iout += 1;
-    }
-    R vt[3];
-    s.template U<ALG>(p, s.v, -0.5 * dt, t0, vt);             // :514
-#pragma unroll
-    for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
-
-    const long long nt = p.nt;
-    long long it = 1;
-    long long next_save = p.save_every ? p.K + 1 : nt + 2;    // first it with (it-1) % K == 0, it-1 > 0
-    double t = fma(-0.5, dt, t0);                             // :451
-    for (; it <= nt; ++it) {
-        R vprev[3] = {s.v[0], s.v[1], s.v[2]};
-        t += dt;                                              // :454
-        R Ec[3], Bc[3];
-        s.eval(p, t, Ec, Bc);
-        s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vt);           // :455
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
-        typename S::Pos xn;
-        do_step(p, s, s.v, dt, xn);                           // :457
-        if (outside(p, s, xn, s.v)) {                         // :459-461
-#pragma unroll
-            for (int c = 0; c < 3; ++c) s.v[c] = vprev[c];
-            break;
-        }
-        if (it == next_save) {                                // :463-472
-            next_save += p.K;
-            iout += 1;
-            if (iout <= p.nout) {
-                const double tc = fma(-0.5, dt, t);
-                R vs[3];
-                // same position as the step's evaluation: static fields are re-used
-                if (p.static_fields) s.template Uf<ALG>(p, Ec, Bc, vprev, 0.5 * dt, vs);
-                else s.template U<ALG>(p, vprev, 0.5 * dt, tc, vs);
-                record<REC>(p, s, i, iout - 1, vs, tc);
-                tlast = tc;
-            }
-        }
-        s.pos = xn;                                           // :474
-    }
-    const long long final_step = it - 1;
-    bool should = p.save_end ||
-                  (p.save_every && final_step > 0 && final_step % p.K == 0);     // :522-528
-    if (iout < p.nout && should) {                                            // :530-543
-        const double tf = (final_step == nt) ? t1 : fma((double)final_step, dt, t0);
-        if (iout == 0 || tlast < tf) {
-            const double dtf = tf - fma((double)final_step - 0.5, dt, t0);
-            R vf[3];
-            s.template U<ALG>(p, s.v, dtf, tf, vf);
-            record<REC>(p, s, i, iout, vf, tf);
-            iout += 1;
-        }
-    }
-    p.out_len[i] = (int)iout;                                                 // :545-551
-    p.retcode[i] = (iout < p.nout) ? kRetTerminated : kRetSuccess;
-    return final_step;
-}
-
-// _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266
-template <class S, int ALG, bool REC>
-__device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
-{
-    using R = typename S::R;
-    S s;
-    s.load(p, i);
-    const double t0 = p.t0, t1 = p.t1;
-    const double C = p.C;                                     // :164
-    const double ttotal = t1 - t0;
-    double t = t0;
-    long long nrec = 0;
-    double tlast = 0.0;
-    R vt[3];
-    auto push = [&](const R vv[3], double tt) {
-        if (nrec < p.cap) record<REC>(p, s, i, nrec, vv, tt);
-        tlast = tt;
-        nrec += 1;
-    };
-    if (p.save_start) push(s.v, t);                           // :179-186
-    // Ec, Bc: the fields at the current (x, t).  The reference evaluates
-    // there up to four times per step (|B| for dt, both halves of the
-    // resynchronisation, a save); one evaluation serves them all, and with
-    // static fields the mid-step update as well.
-    R Ec[3], Bc[3];
-    s.eval(p, t, Ec, Bc);
-    double dt = C / s.bmag_of(Bc);                            // :188-189
-    s.set_dt(dt);
-    s.template Uf<ALG>(p, Ec, Bc, s.v, -0.5 * dt, vt);        // :192
-#pragma unroll
-    for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
-    long long it = 1;
-    long long next_save = p.save_every ? p.K + 1 : -1;
-    bool should = p.save_end;
-    int ret = kRetSuccess;
-    // every wave must leave the loop: a zero step (|B| = inf) or an
-    // unbounded run stops with MaxIters instead of spinning
-    const long long cap_steps = p.max_steps > 0 ? p.max_steps : kAdaptiveHardCap;
-    while (fabs(t - t0) < fabs(ttotal)) {                     // :197
-        if (dt == 0.0) {
-            ret = kRetMaxIters;
-            break;
-        }
-        if (fabs(t + dt - t0) > fabs(ttotal)) {               // :198-203
-            const double dts = t1 - t;
-            s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
-            s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dts, s.v);
-            dt = dts;
-            s.set_dt(dt);
-        }
-        if (it == next_save) {                                // :205-218
-            next_save += p.K;
-            s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
-            push(vt, t);
-        }
-        const double tmid = t + 0.5 * dt;                     // :220
-        R vn[3];
-        if (p.static_fields) s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vn);   // :221
-        else s.template U<ALG>(p, s.v, dt, tmid, vn);
-        typename S::Pos xn;
-        do_step(p, s, vn, dt, xn);                            // :223
-        const double tn = t + dt;
-        if (outside(p, s, xn, vn)) {                          // :227-231
-            should = true;
-            ret = kRetTerminated;
-            break;
-        }
-        s.pos = xn;                                           // :233-235
-        t = tn;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s.v[c] = vn[c];
-        s.eval(p, t, Ec, Bc);
-        const double dtn = C / s.bmag_of(Bc);                 // :237-238
-        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);     // :241-242
-        s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dtn, s.v);
-        dt = dtn;
-        s.set_dt(dt);
-        it += 1;
-        if (it == next_save) should = true;                   // :245-247, (it-1) % K == 0
-        if (it - 1 >= cap_steps) {
-            ret = kRetMaxIters;
-            break;
-        }
-    }
-    if (should && (nrec == 0 || tlast != t)) {                // :250-259
-        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
-        push(vt, t);
-    }
-    p.out_len[i] = (int)nrec;
-    p.retcode[i] = ret;
-    return it - 1;
-}
-
-__device__ __forceinline__ void add_steps(const KParams& p, long long steps)
-{
-    // wave-level sum, then one atomic per wave
-    unsigned long long v = (unsigned long long)steps;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(p.steps, v);
-}
-
-template <class S, int ALG, bool REC = false>
-__global__ void __launch_bounds__(256) k_fixed(const KParams p)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long steps = 0;
-    if (i < p.N) steps = run_fixed<S, ALG, REC>(p, i);
-    add_steps(p, steps);
-}
-
-template <class S, int ALG, bool REC = false>
-__global__ void __launch_bounds__(256) k_adaptive(const KParams p)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long steps = 0;
-    if (i < p.N) steps = run_adaptive<S, ALG, REC>(p, i);
-    add_steps(p, steps);
-}
-
-// ---------------------------------------------------------------------------
-// Specialised gridded-B policies (see tpamd_grid.h).  NS = 0 is the Boris
-// update, NS >= 1 the rational multistep update with n = NS subcycles.
-// ---------------------------------------------------------------------------
-template <int EMODE, int NS, bool PER, typename T, bool CELLS>
-struct S64G : S64 {
-    static constexpr int kNS = NS;
-    __device__ __forceinline__ void bfield(const KParams& p, double B[3]) const
-    {
-        int i0[3], i1[3];
-        double w[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double s = (pos.x[d] - p.B.origin[d]) * p.B.inv_h[d];
-            if (!locate_f64<PER>(p.B, d, s, i0[d], i1[d], w[d])) {
-                B[0] = B[1] = B[2] = __builtin_nan("");
-                return;
-            }
-        }
-        if constexpr (CELLS) {
-            trilinear_cell<T, double>(p.B.cells, cell_offset(p.B, i0), w, B);
-        } else {
-            unsigned ox[2], oy[2], oz[2];
-            offsets(p.B, i0, i1, sizeof(T), ox, oy, oz);
-            trilinear<T, double>(p.B.data, ox, oy, oz, w, B);
-        }
-    }
-    __device__ __forceinline__ void eval(const KParams& p, double, double E[3], double B[3]) const
-    {
-        bfield(p, B);
-        if (EMODE == kEUxB) cross3(B, p.E.v, E);
-        else { E[0] = p.E.v[0]; E[1] = p.E.v[1]; E[2] = p.E.v[2]; }
-    }
-    template <int ALG>
-    __device__ __forceinline__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
-                       double h, double out[3]) const
-    {
-        if (NS == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
-        else multistep_f64_n<NS == 0 ? 1 : NS>(p, vin, E, B, h, out);
-    }
-    template <int ALG>
-    __device__ __forceinline__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
-    {
-        double E[3], B[3];
-        eval(p, t, E, B);
-        Uf<ALG>(p, E, B, vin, h, out);
-    }
-};
-
-template <int EMODE, int NS, bool PER, bool CELLS>
-struct S32G : S32C {
-    static constexpr int kNS = NS;
-    __device__ __forceinline__ void bfield(const KParams& p, float B[3]) const
-    {
-        int i0[3], i1[3];
-        float w[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const float s = fmaf(pos.hi[d], p.B.inv_hf[d], pos.lo[d] * p.B.inv_hf[d]);
-            if (!locate_f32<PER>(p.B, d, s, i0[d], i1[d], w[d])) {
-                B[0] = B[1] = B[2] = __builtin_nanf("");
-                return;
-            }
-        }
-        if constexpr (CELLS) {
-            trilinear_cell<float, float>(p.B.cells, cell_offset(p.B, i0), w, B);
-        } else {
-            unsigned ox[2], oy[2], oz[2];
-            offsets(p.B, i0, i1, 4u, ox, oy, oz);
-            trilinear<float, float>(p.B.data, ox, oy, oz, w, B);
-        }
-    }
-    __device__ __forceinline__ void eval(const KParams& p, double, float E[3], float B[3]) const
-    {
-        bfield(p, B);
-        if (EMODE == kEUxB) cross3f(B, p.f32.u, E);
-        else { E[0] = p.f32.e0[0]; E[1] = p.f32.e0[1]; E[2] = p.f32.e0[2]; }
-    }
-    template <int ALG>
-    __device__ __forceinline__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
-                       double h, float out[3]) const
-    {
-        if (NS == 0) boris_f32_fast(vin, E, B, (float)((p.q2m * 0.5) * h), out);
-        else multistep_f32_n<NS == 0 ? 1 : NS>(p, vin, E, B, (float)(p.q2m_2n * h), out);
-    }
-    template <int ALG>
-    __device__ __forceinline__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
-    {
-        float E[3], B[3];
-        eval(p, t, E, B);
-        Uf<ALG>(p, E, B, vin, h, out);
-    }
-};
-
-// ---------------------------------------------------------------------------
-// Adjoint (see tpamd_adjoint.h).  The forward pass reuses the specialised
-// Boris policies, so its trajectory is bit-identical to a solve of the same
-// problem; FP64 checkpoints are taken every K steps.
-// ---------------------------------------------------------------------------
-// FP32C: float storage (the host checks it); FP64: storage T.  CELLS: the
-// gathers read the grid's cell-replicated copy (one line per gather).
-template <bool F32, bool PER, typename T = double, bool CELLS = false>
-using AdjState = typename std::conditional<F32, S32G<kEUniform, 0, PER, CELLS>,
-                                           S64G<kEUniform, 0, PER, T, CELLS>>::type;
-
-template <bool F32, bool PER, typename T, bool CELLS>
-__device__ __forceinline__ void adj_ckpt_store(const AdjParams& a, long long N, int seg, long long i,
-                                               const AdjState<F32, PER, T, CELLS>& s)
-{
-    double* c = a.ckpt + (size_t)seg * 9 * N + i;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if constexpr (F32) {
-            c[(size_t)d * N] = (double)s.pos.hi[d] + (double)s.pos.lo[d];
-            c[(size_t)(3 + d) * N] = (double)s.pos.wrap[d];
-        } else {
-            c[(size_t)d * N] = s.pos.x[d];
-            c[(size_t)(3 + d) * N] = 0.0;
-        }
-        c[(size_t)(6 + d) * N] = (double)s.v[d];
-    }
-}
-
-template <bool F32, bool PER, typename T, bool CELLS>
-__device__ __forceinline__ void adj_ckpt_load(const AdjParams& a, long long N, int seg, long long i,
-                                              AdjState<F32, PER, T, CELLS>& s)
-{
-    const double* c = a.ckpt + (size_t)seg * 9 * N + i;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double r = c[(size_t)d * N];
-        if constexpr (F32) {
-            s.pos.hi[d] = (float)r;
-            s.pos.lo[d] = (float)(r - (double)s.pos.hi[d]);
-            s.pos.wrap[d] = (int)c[(size_t)(3 + d) * N];
-        } else {
-            s.pos.x[d] = r;
-        }
-        s.v[d] = (typename AdjState<F32, PER, T, CELLS>::R)c[(size_t)(6 + d) * N];
-    }
-}
-
-// cell coordinates of the current position, as the forward gather computes them
-template <bool F32, bool PER, typename T, bool CELLS>
-__device__ __forceinline__ void adj_cellcoord(const KParams& p, const AdjState<F32, PER, T, CELLS>& s,
-                                              typename AdjState<F32, PER, T, CELLS>::R sc[3])
-{
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if constexpr (F32) sc[d] = fmaf(s.pos.hi[d], p.B.inv_hf[d], s.pos.lo[d] * p.B.inv_hf[d]);
-        else sc[d] = (s.pos.x[d] - p.B.origin[d]) * p.B.inv_h[d];
-    }
-}
-
-// a record that adds nothing, keyed to a cell that spreads such records over
-// the bins
-template <typename R>
-__device__ __forceinline__ void adj_zero_record(const AdjParams& a, size_t rec)
-{
-    a.rec_key[rec] = (unsigned)(rec % (size_t)a.ncell);
-    AdjRec<R> v;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        v.bb[k] = (R)0;
-        v.w[k] = (R)0;
-    }
-    ((AdjRec<R>*)a.rec_val)[rec] = v;
-}
-
-// one reverse step at the state held in s: (xbar, ubar) <- VJP of the step
-// that started there, with rotation factor h and position step dt.  The
-// step's dL/dB contribution goes to record slot `rec` (cell, Bbar, weights).
-template <bool F32, bool PER, typename T, bool CELLS>
-__device__ __forceinline__ void adj_step_back(const KParams& p, const AdjParams& a,
-                                              const AdjState<F32, PER, T, CELLS>& s, const typename AdjState<F32, PER, T, CELLS>::R u[3],
-                                              double h, double dt, typename AdjState<F32, PER, T, CELLS>::R xb[3],
-                                              typename AdjState<F32, PER, T, CELLS>::R ub[3], bool with_dt,
-                                              size_t rec)
-{
-    using R = typename AdjState<F32, PER, T, CELLS>::R;
-    R sc[3], w[3], c[8][3], B[3];
-    int i0[3], i1[3];
-    unsigned node[8];
-    adj_cellcoord<F32, PER, T, CELLS>(p, s, sc);
-    if (!adj_locate<PER, R>(p.B, sc, i0, i1, w)) {
-        adj_zero_record<R>(a, rec);
-        return;
-    }
-    if constexpr (CELLS) adj_corners_cell<T, R>(p.B, i0, c);
-    else adj_corners<T, R>(p.B, i0, i1, c, node);
-    adj_value<R>(c, w, B);
-    const R hh = (R)h;
-    R t[3], g[3], tb[3], un[3], Bb[3], xg[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        t[k] = hh * B[k];
-        g[k] = with_dt ? ub[k] + (R)dt * xb[k] : ub[k];
-    }
-    boris_vjp<R>(u, t, g, un, tb);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) Bb[k] = hh * tb[k];
-    adj_xgrad<R>(p.B, c, w, Bb, xg);
-    if constexpr (!PER) {
-        // outside the node box a clamped field does not change along that axis
-        if (p.B.extrap == kExtrapClamp) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                if (!(sc[k] >= (R)0 && sc[k] <= (R)(p.B.n[k] - 1))) xg[k] = (R)0;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        xb[k] += xg[k];
-        ub[k] = un[k];
-    }
-    a.rec_key[rec] = (unsigned)i0[0] + (unsigned)a.nc[0] * ((unsigned)i0[1] + (unsigned)a.nc[1] * (unsigned)i0[2]);
-    AdjRec<R> v;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        v.bb[k] = Bb[k];
-        v.w[k] = w[k];
-    }
-    ((AdjRec<R>*)a.rec_val)[rec] = v;
-}
-
-template <bool F32, bool PER, typename T, bool CELLS>
-__global__ void __launch_bounds__(256) k_adj_forward(const KParams p, const AdjParams a)
-{
-    using S = AdjState<F32, PER, T, CELLS>;
-    using R = typename S::R;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    double lsum = 0.0;
-    if (i < p.N) {
-        S s;
-        s.load(p, i);
-        const double dt = p.dt;
-        s.set_dt(dt);
-        R vt[3];
-        s.template U<0>(p, s.v, -0.5 * dt, p.t0, vt);            // boris.jl:514
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
-        double t = fma(-0.5, dt, p.t0);
-        for (long long n = 0; n < p.nt; ++n) {
-            if (n % a.K == 0) adj_ckpt_store<F32, PER, T, CELLS>(a, p.N, (int)(n / a.K), i, s);
-            t += dt;
-            s.template U<0>(p, s.v, dt, t, vt);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
-            typename S::Pos xn;
-            do_step(p, s, s.v, dt, xn);
-            s.pos = xn;
-        }
-        double x[3];
-        s.abs_pos(p, s.pos, x);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            a.xfinal[3 * i + c] = x[c];
-            const double d = x[c] - a.targets[3 * i + c];
-            lsum += d * d;
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
-    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(a.loss, lsum / (double)p.N);
-}
-
-// Reverse sweep over segment `seg` (launched for seg = nseg-1 .. 0): reload
-// the adjoint state, recompute the segment from its checkpoint, walk it
-// backwards writing one record per step into rows 0..K-1 of the record
-// buffer (row K holds the t0 half-step on segment 0; unused rows are marked
-// invalid), and store the adjoint state for the next launch.
-template <bool F32, bool PER, typename T, bool CELLS>
-__global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjParams a, int seg)
-{
-    using S = AdjState<F32, PER, T, CELLS>;
-    using R = typename S::R;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.N) return;
-    const long long N = p.N;
-    const double dt = p.dt;
-    const double h = (p.q2m * 0.5) * dt;
-    R* st = (R*)a.state;
-    R xb[3], ub[3];
-    if (seg == a.nseg - 1) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xb[c] = (R)(2.0 / (double)N * (a.xfinal[3 * i + c] - a.targets[3 * i + c]));
-            ub[c] = (R)0;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xb[c] = st[(size_t)c * N + i];
-            ub[c] = st[(size_t)(3 + c) * N + i];
-        }
-    }
-    constexpr int NC = F32 ? 9 : 6;
-    R* buf = (R*)a.seg;
-    const long long n0 = (long long)seg * a.K;
-    const int len = (int)((p.nt - n0) < a.K ? (p.nt - n0) : a.K);
-    S s;
-    adj_ckpt_load<F32, PER, T, CELLS>(a, N, seg, i, s);
-    s.set_dt(dt);
-    R vt[3];
-    for (int j = 0; j < len; ++j) {                           // recompute the segment
-        R* b = buf + (size_t)j * NC * N + i;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if constexpr (F32) {
-                b[(size_t)d * N] = s.pos.hi[d];
-                b[(size_t)(3 + d) * N] = s.pos.lo[d];
-                b[(size_t)(6 + d) * N] = s.v[d];
-            } else {
-                b[(size_t)d * N] = s.pos.x[d];
-                b[(size_t)(3 + d) * N] = s.v[d];
-            }
-        }
-        s.template U<0>(p, s.v, dt, 0.0, vt);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
-        typename S::Pos xn;
-        do_step(p, s, s.v, dt, xn);
-        s.pos = xn;
-    }
-    for (int j = a.K - 1; j >= len; --j) adj_zero_record<R>(a, (size_t)j * N + i);
-    for (int j = len - 1; j >= 0; --j) {                      // and walk it backwards
-        const R* b = buf + (size_t)j * NC * N + i;
-        R u[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if constexpr (F32) {
-                s.pos.hi[d] = b[(size_t)d * N];
-                s.pos.lo[d] = b[(size_t)(3 + d) * N];
-                u[d] = b[(size_t)(6 + d) * N];
-            } else {
-                s.pos.x[d] = b[(size_t)d * N];
-                u[d] = b[(size_t)(3 + d) * N];
-            }
-        }
-        adj_step_back<F32, PER, T, CELLS>(p, a, s, u, h, dt, xb, ub, true, (size_t)j * N + i);
-    }
-    const size_t last = (size_t)a.K * N + i;
-    if (seg == 0) {
-        // the initial back half-step u(-1/2) = U(v0, x0, -dt/2) (boris.jl:514)
-        S s0;
-        s0.load(p, i);
-        R v0[3] = {s0.v[0], s0.v[1], s0.v[2]};
-        adj_step_back<F32, PER, T, CELLS>(p, a, s0, v0, (p.q2m * 0.5) * (-0.5 * dt), 0.0, xb, ub, false, last);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            a.grad_xv0[6 * i + c] = (double)xb[c];
-            a.grad_xv0[6 * i + 3 + c] = (double)ub[c];
-        }
-    } else {
-        adj_zero_record<R>(a, last);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            st[(size_t)c * N + i] = xb[c];
-            st[(size_t)(3 + c) * N + i] = ub[c];
-        }
-    }
-}
-
-__global__ void k_zero_f32(float* p, long long n)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
-}
-
-__global__ void k_zero_f64(double* p, long long n)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
-}
-
-// ---------------------------------------------------------------------------
-// Batched ensembles: a persistent grid of waves pulls chunks of particles from
-// an atomic queue.  A chunk never straddles a field instance, so the
-// instance's parameter block is wave-uniform; within a chunk the 64 lanes
-// stride over its particles (particles per thread = chunk size / 64), which
-// evens out uneven lifetimes.  Every wave leaves once the queue is drained.
-// ---------------------------------------------------------------------------
-struct Chunk {
-    int inst;
-    int pad;
-    long long begin, end;
-};
 
 __global__ void k_batch_reset(unsigned* counter, unsigned long long* steps)
 {
@@ -872,38 +26,6 @@ __global__ void k_batch_reset(unsigned* counter, unsigned long long* steps)
         *counter = 0u;
         *steps = 0ull;
     }
-}
-
-template <class S, int ALG, bool ADAPT, bool REC = false>
-__global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table,
-                                               const Chunk* __restrict__ chunks, int nchunks,
-                                               unsigned* __restrict__ counter,
-                                               unsigned long long* __restrict__ steps_out)
-{
-    const int lane = threadIdx.x & 63;
-    long long steps = 0;
-    // the chunk index is made wave-uniform (readfirstlane), so the chunk and
-    // the instance's parameter block are read with scalar loads; the next
-    // chunk is claimed before the current one is worked on, which hides the
-    // atomic's round trip
-    unsigned claim = 0;
-    if (lane == 0) claim = atomicAdd(counter, 1u);
-    unsigned c = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
-    while (c < (unsigned)nchunks) {
-        unsigned next = 0;
-        if (lane == 0) next = atomicAdd(counter, 1u);
-        const Chunk ch = chunks[c];
-        const KParams& p = table[ch.inst];
-        for (long long i = ch.begin + lane; i < ch.end; i += 64) {
-            if constexpr (ADAPT) steps += run_adaptive<S, ALG, REC>(p, i);
-            else steps += run_fixed<S, ALG, REC>(p, i);
-        }
-        c = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
-    }
-    unsigned long long v = (unsigned long long)steps;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) atomicAdd(steps_out, v);
 }
 
 __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const double* __restrict__ x,
@@ -944,7 +66,7 @@ __global__ void __launch_bounds__(256) k_record_eval(const KParams p, const doub
         x[c] = xv[6 * i + c];
         v[c] = xv[6 * i + 3 + c];
     }
-    record_extra<true>(p, x, v, t[i], r);
+    record_extra<kFkMesh>(p, x, v, t[i], r);
     const int w = p.rec_w - 6;
     for (int c = 0; c < w; ++c) out[(size_t)w * i + c] = r[6 + c];
 }
@@ -954,31 +76,13 @@ __global__ void __launch_bounds__(256) k_record_eval(const KParams p, const doub
 // ===========================================================================
 // Host side
 // ===========================================================================
-struct tpamd_field {
-    DevField d;
-    void* dev_data;          // owned device buffer (grids)
-    void* cells_data;        // owned cell-replicated copy (grids, optional)
-    const tpamd_field* src;  // UxB: the B field it was built from
-    size_t bytes;
-    MeshGrid mesh_host;      // Mesh: host copy of the descriptor (device pointers)
-};
-
-struct tpamd_ctx {
-    int device;
-    hipStream_t stream;
-    hipEvent_t ev0, ev1;
-    std::string err;
-    // staging buffers for tpamd_solve (grown on demand)
-    void* d_buf;
-    size_t d_buf_bytes;
-    unsigned long long* d_steps;
-};
-
 namespace {
 
 thread_local std::string g_init_err;
 
-int fail(tpamd_ctx* ctx, int code, const char* fmt, ...)
+}  // namespace
+
+int tpamd::fail(tpamd_ctx* ctx, int code, const char* fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -990,13 +94,8 @@ int fail(tpamd_ctx* ctx, int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(ctx, expr)                                                                   \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? TPAMD_ERR_OOM : TPAMD_ERR_DEVICE, \
-                        "%s failed: %s", #expr, hipGetErrorString(e_));                     \
-    } while (0)
+namespace {
+
 
 int fill_poly(int n, Poly& P)
 {
@@ -1077,8 +176,12 @@ int check_desc(tpamd_ctx* ctx, const tpamd_solve_desc* d, int64_t* nt, int64_t* 
 
 bool is_gridded(const DevField& f) { return f.kind == kFieldGrid || f.kind == kFieldTimeGrid; }
 
-// the solve's E or B is a mesh: the generic kernels must carry the mesh evaluator
-bool has_mesh(const KParams& p) { return p.B.kind == kFieldMesh || (!p.e_is_uxb && p.E.kind == kFieldMesh); }
+// the field-kind class of a solve: the generic kernels compiled for it
+int solve_class(const KParams& p)
+{
+    const int b = field_class(p.B.kind);
+    return p.e_is_uxb ? b : std::max(b, field_class(p.E.kind));
+}
 
 bool same_geometry(const DevField& a, const DevField& b)
 {
@@ -1130,42 +233,12 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
 }
 
 // ---------------------------------------------------------------------------
-// Policy selection.  select_policy calls l.go<S, ALG>() with the specialised
-// gridded-B policy that matches the solve, or with the generic policy.  The
-// specialisations cover: B a grid whose bytes fit 32-bit offsets, E zero,
-// uniform or UxB, the Boris update (fixed or adaptive) or the rational
-// multistep update with n <= 2 (fixed step), FP32C on float storage.
+// Policy selection: the specialised gridded-B kernels where they apply, the
+// generic ones otherwise.  The specialisations cover: B a grid whose bytes
+// fit 32-bit offsets, E zero, uniform or UxB, the Boris update (fixed or
+// adaptive) or the rational multistep update with n <= 2 (fixed step), FP32C
+// on float storage.
 // ---------------------------------------------------------------------------
-template <class L, int EMODE, int NS, bool CELLS>
-void sel_g32(const L& l, bool per)
-{
-    if (per) l.template go<S32G<EMODE, NS, true, CELLS>, NS == 0 ? 0 : 1>();
-    else l.template go<S32G<EMODE, NS, false, CELLS>, NS == 0 ? 0 : 1>();
-}
-
-template <class L, int EMODE, int NS, typename T, bool CELLS>
-void sel_g64(const L& l, bool per)
-{
-    if (per) l.template go<S64G<EMODE, NS, true, T, CELLS>, NS == 0 ? 0 : 1>();
-    else l.template go<S64G<EMODE, NS, false, T, CELLS>, NS == 0 ? 0 : 1>();
-}
-
-template <class L, int EMODE, bool CELLS>
-void sel_g32_ns(const L& l, int ns, bool per)
-{
-    if (ns == 0) sel_g32<L, EMODE, 0, CELLS>(l, per);
-    else if (ns == 1) sel_g32<L, EMODE, 1, CELLS>(l, per);
-    else sel_g32<L, EMODE, 2, CELLS>(l, per);
-}
-
-template <class L, int EMODE, typename T, bool CELLS>
-void sel_g64_ns(const L& l, int ns, bool per)
-{
-    if (ns == 0) sel_g64<L, EMODE, 0, T, CELLS>(l, per);
-    else if (ns == 1) sel_g64<L, EMODE, 1, T, CELLS>(l, per);
-    else sel_g64<L, EMODE, 2, T, CELLS>(l, per);
-}
-
 // -1 when no specialisation applies, else the subcycle class NS (0 = Boris)
 int spec_class(const tpamd_solve_desc* d, const KParams& p, int* emode)
 {
@@ -1187,123 +260,28 @@ int spec_class(const tpamd_solve_desc* d, const KParams& p, int* emode)
     return ns;
 }
 
-template <class L>
-void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec, bool mesh, const L& l)
+void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec, int fk,
+                   const LaunchReq& r)
 {
     int emode = 0;
     const int ns = allow_spec ? spec_class(d, p, &emode) : -1;
     if (ns >= 0) {
         const bool per = p.B.extrap == kExtrapPeriodic;
         const bool cells = p.B.cells != nullptr;
-        if (d->precision == TPAMD_FP32C) {
-            if (cells) {
-                if (emode == kEUxB) sel_g32_ns<L, kEUxB, true>(l, ns, per);
-                else sel_g32_ns<L, kEUniform, true>(l, ns, per);
-            } else {
-                if (emode == kEUxB) sel_g32_ns<L, kEUxB, false>(l, ns, per);
-                else sel_g32_ns<L, kEUniform, false>(l, ns, per);
-            }
-        } else if (p.B.dtype == 1) {
-            if (emode == kEUxB) sel_g64_ns<L, kEUxB, float, false>(l, ns, per);
-            else sel_g64_ns<L, kEUniform, float, false>(l, ns, per);
-        } else if (cells) {
-            if (emode == kEUxB) sel_g64_ns<L, kEUxB, double, true>(l, ns, per);
-            else sel_g64_ns<L, kEUniform, double, true>(l, ns, per);
-        } else {
-            if (emode == kEUxB) sel_g64_ns<L, kEUxB, double, false>(l, ns, per);
-            else sel_g64_ns<L, kEUniform, double, false>(l, ns, per);
-        }
+        if (d->precision == TPAMD_FP32C) launch_spec32(r, emode, ns, per, cells);
+        else launch_spec64(r, emode, ns, per, p.B.dtype == 1, cells);
         return;
     }
     if (d->precision == TPAMD_FP64) {
-        if (mesh) {
-            if (p.alg == 0) l.template go<S64M, 0>();
-            else l.template go<S64M, 1>();
-        } else {
-            if (p.alg == 0) l.template go<S64, 0>();
-            else l.template go<S64, 1>();
-        }
+        if (fk == kFkMesh) launch_generic64_mesh(r, p.alg);
+        else if (fk == kFkGrid) launch_generic64(r, p.alg);
+        else launch_generic64_analytic(r, p.alg);
     } else {
-        if (mesh) {
-            if (p.alg == 0) l.template go<S32CM, 0>();
-            else l.template go<S32CM, 1>();
-        } else {
-            if (p.alg == 0) l.template go<S32C, 0>();
-            else l.template go<S32C, 1>();
-        }
+        if (fk == kFkMesh) launch_generic32_mesh(r, p.alg);
+        else if (fk == kFkGrid) launch_generic32(r, p.alg);
+        else launch_generic32_analytic(r, p.alg);
     }
 }
-
-// one solve: a thread per particle
-struct SoloLaunch {
-    const KParams& p;
-    int adaptive;
-    dim3 g, b;
-    hipStream_t st;
-    template <class S, int ALG>
-    void go() const
-    {
-        // records with E, B or work columns: generic policies only (spec_class
-        // declines them), in the REC instantiations
-        if constexpr (S::kNS < 0) {
-            if (p.rec_w > 6) {
-                if (adaptive) hipLaunchKernelGGL((k_adaptive<S, ALG, true>), g, b, 0, st, p);
-                else hipLaunchKernelGGL((k_fixed<S, ALG, true>), g, b, 0, st, p);
-                return;
-            }
-        }
-        if constexpr (S::kNS <= 0) {
-            if (adaptive) {
-                hipLaunchKernelGGL((k_adaptive<S, ALG>), g, b, 0, st, p);
-                return;
-            }
-        }
-        hipLaunchKernelGGL((k_fixed<S, ALG>), g, b, 0, st, p);
-    }
-};
-
-// batched solve: persistent waves over the chunk queue.  With `occ` set it
-// only reports the kernel's resident blocks per CU for block size b.x.
-struct BatchLaunch {
-    const KParams* table;
-    const Chunk* chunks;
-    int nchunks;
-    unsigned* counter;
-    unsigned long long* steps;
-    int adaptive;
-    dim3 g, b;
-    hipStream_t st;
-    int* occ;
-    bool rec;                 // records carry E, B or work columns
-    template <class K>
-    void run(K kern) const
-    {
-        if (occ) {
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, kern, (int)b.x, 0) != hipSuccess)
-                *occ = 1;
-            return;
-        }
-        hipLaunchKernelGGL(kern, g, b, 0, st, table, chunks, nchunks, counter, steps);
-    }
-    template <class S, int ALG>
-    void go() const
-    {
-        if constexpr (S::kNS < 0) {
-            if (rec) {
-                if (adaptive) run(k_batch<S, ALG, true, true>);
-                else run(k_batch<S, ALG, false, true>);
-                return;
-            }
-        }
-        if constexpr (S::kNS <= 0) {
-            if (adaptive) {
-                run(k_batch<S, ALG, true>);
-                return;
-            }
-        }
-        run(k_batch<S, ALG, false>);
-    }
-};
 
 // Validates the descriptor and fills the kernel parameter block (everything
 // but the launch).  Shared by single and batched solves.
@@ -1422,7 +400,14 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     if (N > 0) {
         const dim3 block(bs), grid((unsigned)((N + bs - 1) / bs));
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-        select_policy(desc, p, true, has_mesh(p), SoloLaunch{p, desc->adaptive, grid, block, st});
+        LaunchReq r{};
+        r.batch = false;
+        r.adaptive = desc->adaptive;
+        r.g = grid;
+        r.b = block;
+        r.st = st;
+        r.p = &p;
+        select_policy(desc, p, true, solve_class(p), r);
         HIP_TRY(ctx, hipGetLastError());
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     }
@@ -1462,7 +447,7 @@ struct tpamd_batch {
     unsigned* d_counter;
     unsigned long long* d_steps;
     bool allow_spec;
-    bool any_mesh;
+    int fk;                  // field-kind class covering every instance
     int block_size;
     int ppt;
     int n_cu;
@@ -1509,9 +494,20 @@ int batch_enqueue(tpamd_batch* b, hipStream_t st)
     const dim3 block(b->block_size);
     int occ = 0;
     const bool rec = TPAMD_RECORD_WIDTH(&b->desc) > 6;
-    BatchLaunch q{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
-                  dim3(1), block, st, &occ, rec};
-    select_policy(&b->desc, b->table[0], b->allow_spec, b->any_mesh, q);
+    LaunchReq q{};
+    q.batch = true;
+    q.adaptive = b->desc.adaptive;
+    q.g = dim3(1);
+    q.b = block;
+    q.st = st;
+    q.table = b->d_table;
+    q.chunks = b->d_chunks;
+    q.nchunks = b->nchunks;
+    q.counter = b->d_counter;
+    q.steps = b->d_steps;
+    q.occ = &occ;
+    q.rec = rec;
+    select_policy(&b->desc, b->table[0], b->allow_spec, b->fk, q);
     occ = std::max(occ, 1);
     const long long waves_needed = b->nchunks;
     // The occupancy query counts 256 VGPRs per lane, half of what gfx950
@@ -1523,9 +519,10 @@ int batch_enqueue(tpamd_batch* b, hipStream_t st)
     blocks = std::min(blocks, (waves_needed + waves_per_block - 1) / waves_per_block);
     blocks = std::max(blocks, 1LL);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counter, b->d_steps);
-    BatchLaunch l{b->d_table, b->d_chunks, b->nchunks, b->d_counter, b->d_steps, b->desc.adaptive,
-                  dim3((unsigned)blocks), block, st, nullptr, rec};
-    select_policy(&b->desc, b->table[0], b->allow_spec, b->any_mesh, l);
+    LaunchReq l = q;
+    l.g = dim3((unsigned)blocks);
+    l.occ = nullptr;
+    select_policy(&b->desc, b->table[0], b->allow_spec, b->fk, l);
     HIP_TRY(ctx, hipGetLastError());
     return TPAMD_OK;
 }
@@ -1571,130 +568,6 @@ int batch_time_once(tpamd_batch* b, hipStream_t st, float* ms)
 // first) the reverse sweep, the radix partition of its records by cell bin
 // and the per-bin accumulation; finally the fold of per-cell corner moments
 // into the node gradient.  R is the working precision (float for FP32C).
-template <bool F32, bool PER, typename T, bool CELLS>
-int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* xv0, int64_t N,
-                const double* targets, int K, int64_t nt, double* loss, double* dL_dB,
-                double* dL_dxv0, tpamd_stats* stats)
-{
-    using R = typename AdjState<F32, PER, T, CELLS>::R;
-    const int nseg = (int)((nt + K - 1) / K);
-    const int ncomp = F32 ? 9 : 6;
-    const size_t rs = sizeof(R);
-    const int nx = B->d.n[0], ny = B->d.n[1], nz = B->d.n[2];
-    const size_t nnodes = (size_t)nx * ny * nz;
-    int nc[3];
-    for (int k = 0; k < 3; ++k) nc[k] = PER ? B->d.n[k] : B->d.n[k] - 1;
-    if (nc[0] < 1 || nc[1] < 1 || nc[2] < 1)
-        return fail(ctx, TPAMD_ERR_ARGUMENT, "the adjoint's grid needs at least one cell per axis");
-    const double ncell_d = (double)nc[0] * nc[1] * nc[2];
-    if (ncell_d >= 1073741824.0) return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the adjoint supports fewer than 2^30 cells");
-    const unsigned ncell = (unsigned)ncell_d;
-    const long long M = (long long)(K + 1) * N;              // record rows 0..K
-    if ((double)(K + 1) * (double)N >= 2147483648.0)
-        return fail(ctx, TPAMD_ERR_CAPACITY, "too many adjoint records: lower the checkpoint interval or N");
-    // records are partitioned on the bin bits [kShift, cellbits) of the cell
-    // index; a grid of one bin needs no partition
-    constexpr int kShift = AdjBin::kShift;
-    int cellbits = 0;
-    while ((1u << cellbits) < ncell) ++cellbits;
-    const bool partition = cellbits > kShift;
-    const unsigned nbins = (ncell + AdjBin::kCells - 1) / AdjBin::kCells;
-    size_t temp_bytes = 0;
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (partition)
-        HIP_TRY(ctx, adj_sort_records<R>(nullptr, temp_bytes, nullptr, nullptr, nullptr, nullptr, M, kShift,
-                                         cellbits, ctx->stream));
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t b_xv = al(sizeof(double) * 6 * N), b_tg = al(sizeof(double) * 3 * N),
-                 b_ck = al(sizeof(double) * 9 * (size_t)nseg * N), b_sg = al(rs * ncomp * (size_t)K * N),
-                 b_st = al(rs * 6 * (size_t)N), b_key = al(sizeof(unsigned) * (size_t)M),
-                 b_val = al(sizeof(AdjRec<R>) * (size_t)M), b_tmp = al(std::max<size_t>(temp_bytes, 1)),
-                 b_G = al(sizeof(double) * 24 * (size_t)ncell), b_out = al(sizeof(double) * 3 * nnodes),
-                 b_gx = al(sizeof(double) * 6 * N), b_ls = al(sizeof(double)), b_xf = al(sizeof(double) * 3 * N);
-    const size_t total = b_xv + b_tg + b_ck + b_sg + b_st + 2 * b_key + 2 * b_val + b_tmp + b_G + b_out +
-                         b_gx + b_ls + b_xf;
-    char* buf = nullptr;
-    HIP_TRY(ctx, hipMalloc((void**)&buf, total));
-    char* q = buf;
-    auto take = [&](size_t n) { char* r = q; q += n; return (void*)r; };
-    double* d_xv = (double*)take(b_xv);
-    double* d_tg = (double*)take(b_tg);
-    AdjParams a;
-    a.ckpt = (double*)take(b_ck);
-    a.seg = take(b_sg);
-    a.state = take(b_st);
-    unsigned* keys0 = (unsigned*)take(b_key);
-    unsigned* keys1 = (unsigned*)take(b_key);
-    AdjRec<R>* vals0 = (AdjRec<R>*)take(b_val);
-    AdjRec<R>* vals1 = (AdjRec<R>*)take(b_val);
-    void* d_tmp = take(b_tmp);
-    double* G = (double*)take(b_G);
-    double* d_out = (double*)take(b_out);
-    a.grad_xv0 = (double*)take(b_gx);
-    a.loss = (double*)take(b_ls);
-    a.xfinal = (double*)take(b_xf);
-    a.targets = d_tg;
-    a.rec_key = keys0;
-    a.rec_val = vals0;
-    a.K = K;
-    a.nseg = nseg;
-    a.ncell = ncell;
-    for (int k = 0; k < 3; ++k) a.nc[k] = nc[k];
-    p.xv0 = d_xv;
-    hipStream_t st = ctx->stream;
-    const dim3 block(256), grid((unsigned)((N + 255) / 256));
-    const long long ng = 24 * (long long)ncell;
-    float ms_f = 0.f, ms_r = 0.f;
-    hipError_t e = hipMemcpyAsync(d_xv, xv0, sizeof(double) * 6 * N, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_tg, targets, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, G, ng);
-        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, a.loss, 1LL);
-        e = hipEventRecord(ctx->ev0, st);
-    }
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL((k_adj_forward<F32, PER, T, CELLS>), grid, block, 0, st, p, a);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms_f, ctx->ev0, ctx->ev1);
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, st);
-    for (int seg = nseg - 1; seg >= 0 && e == hipSuccess; --seg) {
-        hipLaunchKernelGGL((k_adj_reverse<F32, PER, T, CELLS>), grid, block, 0, st, p, a, seg);
-        e = hipGetLastError();
-        if (e == hipSuccess && partition)
-            e = adj_sort_records<R>(d_tmp, temp_bytes, keys0, keys1, vals0, vals1, M, kShift, cellbits, st);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL((k_adj_bins<R>), dim3(nbins), dim3(AdjBin::kThreads), 0, st,
-                               partition ? keys1 : keys0, partition ? vals1 : vals0, M, ncell, G);
-            e = hipGetLastError();
-        }
-    }
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_adj_nodes<double>, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, st, G, nx,
-                           ny, nz, nc[0], nc[1], nc[2], PER ? 1 : 0, d_out);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms_r, ctx->ev0, ctx->ev1);
-    if (e == hipSuccess) e = hipMemcpy(dL_dB, d_out, sizeof(double) * 3 * nnodes, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(dL_dxv0, a.grad_xv0, sizeof(double) * 6 * N, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(loss, a.loss, sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(buf);
-    if (e != hipSuccess) return fail(ctx, TPAMD_ERR_DEVICE, "adjoint failed: %s", hipGetErrorString(e));
-    if (stats) {
-        stats->kernel_ms = ms_f;
-        stats->aux_ms = ms_r;
-        stats->particle_steps = nt * N;
-        stats->nt = nt;
-        stats->nout = 1;
-        stats->max_len = 1;
-        stats->total_ms = ms_f + ms_r;
-    }
-    return TPAMD_OK;
-}
 
 }  // namespace
 
@@ -2332,27 +1205,8 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
         return fail(ctx, TPAMD_ERR_UNSUPPORTED, "the adjoint needs a node array below 4 GiB");
     const bool per = B->d.extrap == kExtrapPeriodic;
     const bool cells = B->d.cells != nullptr;      // gathers from the cell-replicated copy
-#define TPAMD_ADJ(F32_, PER_, T_, CELLS_) \
-    return adjoint_run<F32_, PER_, T_, CELLS_>(ctx, p, B, xv0, N, targets, ckpt_interval, nt, loss, dL_dB, dL_dxv0, stats)
-    if (f32) {
-        if (cells) {
-            if (per) TPAMD_ADJ(true, true, float, true);
-            TPAMD_ADJ(true, false, float, true);
-        }
-        if (per) TPAMD_ADJ(true, true, float, false);
-        TPAMD_ADJ(true, false, float, false);
-    }
-    if (bf32) {
-        if (per) TPAMD_ADJ(false, true, float, false);
-        TPAMD_ADJ(false, false, float, false);
-    }
-    if (cells) {
-        if (per) TPAMD_ADJ(false, true, double, true);
-        TPAMD_ADJ(false, false, double, true);
-    }
-    if (per) TPAMD_ADJ(false, true, double, false);
-    TPAMD_ADJ(false, false, double, false);
-#undef TPAMD_ADJ
+    return adjoint_launch(ctx, p, B, xv0, N, targets, ckpt_interval, nt, loss, dL_dB, dL_dxv0, stats,
+                          f32, bf32, per, cells);
 }
 
 int32_t tpamd_batch_free(tpamd_batch* b)
@@ -2420,8 +1274,8 @@ int32_t tpamd_batch_create(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t
     int em0 = 0;
     const int ns0 = spec_class(desc, b->table[0], &em0);
     b->allow_spec = ns0 >= 0;
-    b->any_mesh = false;
-    for (int i = 0; i < n_inst; ++i) b->any_mesh = b->any_mesh || has_mesh(b->table[i]);
+    b->fk = kFkAnalytic;
+    for (int i = 0; i < n_inst; ++i) b->fk = std::max(b->fk, solve_class(b->table[i]));
     for (int i = 1; i < n_inst && b->allow_spec; ++i) {
         int em = 0;
         const int ns = spec_class(desc, b->table[i], &em);

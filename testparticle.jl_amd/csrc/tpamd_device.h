@@ -354,12 +354,22 @@ __device__ __forceinline__ const void* snapshot(const DevField& f, int k)
     return (const char*)f.data + (size_t)k * (size_t)f.snap_bytes;
 }
 
-// MESH instantiations also evaluate mesh fields (out of line); the others
-// leave them out, so kernels that never see a mesh do not carry its call
-template <bool MESH = false>
+// Field-kind classes a generic kernel is compiled for: analytic fields only
+// (zero, uniform, affine, dipole), + uniform grids and their time sequences,
+// + meshes (through the out-of-line mesh evaluator).  A solve takes the
+// smallest class that holds its E and B, so kernels never carry the code
+// (and the registers) of field kinds they cannot meet.
+enum { kFkAnalytic = 0, kFkGrid = 1, kFkMesh = 2 };
+
+__host__ __device__ inline int field_class(int kind)
+{
+    return kind == kFieldMesh ? kFkMesh : ((kind == kFieldGrid || kind == kFieldTimeGrid) ? kFkGrid : kFkAnalytic);
+}
+
+template <int FK = kFkMesh>
 __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], double t, double o[3])
 {
-    if constexpr (MESH) {
+    if constexpr (FK == kFkMesh) {
         if (f.kind == kFieldMesh) {
             const MeshVal r = mesh_value(f.mesh, x[0], x[1], x[2]);
             o[0] = r.v[0]; o[1] = r.v[1]; o[2] = r.v[2];
@@ -391,26 +401,33 @@ __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], 
         return;
     }
     case kFieldGrid:
-        grid_f64(f, x, o);
-        return;
-    case kFieldTimeGrid: {
-        double w;
-        const int k = time_bracket(f, t, w);
-        if (k == 0 || k >= f.nsnap) {
-            grid_f64(f, x, o, snapshot(f, k == 0 ? 0 : f.nsnap - 1));
+        if constexpr (FK >= kFkGrid) {
+            grid_f64(f, x, o);
             return;
         }
-        double a[3], b[3];
-        grid_f64(f, x, a, snapshot(f, k - 1));
-        grid_f64(f, x, b, snapshot(f, k));
+        break;
+    case kFieldTimeGrid:
+        if constexpr (FK >= kFkGrid) {
+            double w;
+            const int k = time_bracket(f, t, w);
+            if (k == 0 || k >= f.nsnap) {
+                grid_f64(f, x, o, snapshot(f, k == 0 ? 0 : f.nsnap - 1));
+                return;
+            }
+            double a[3], b[3];
+            grid_f64(f, x, a, snapshot(f, k - 1));
+            grid_f64(f, x, b, snapshot(f, k));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) o[c] = (1.0 - w) * a[c] + w * b[c];   // interpolation.jl:377
-        return;
+            for (int c = 0; c < 3; ++c) o[c] = (1.0 - w) * a[c] + w * b[c];   // interpolation.jl:377
+            return;
+        }
+        break;
+    default:
+        break;
     }
-    default:  // zero, and meshes outside MESH instantiations
-        o[0] = o[1] = o[2] = 0.0;
-        return;
-    }
+    // zero (and kinds outside this instantiation's class, which the host
+    // never hands it)
+    o[0] = o[1] = o[2] = 0.0;
 }
 
 // slope of the trilinear interpolant at x (snapshot `data`): J[i][j] = dB_i/dx_j;
@@ -492,11 +509,11 @@ __device__ __forceinline__ void grid_jac_f64(const DevField& f, const void* data
 // differentiates the field callables with ForwardDiff; here each field kind
 // has its derivative in closed form.  B itself is field_f64's value.
 // ---------------------------------------------------------------------------
-template <bool MESH = false>
+template <int FK = kFkMesh>
 __device__ __forceinline__ void field_jac_f64(const DevField& f, const double x[3], double t,
                                               double B[3], double J[3][3], double dBdt[3])
 {
-    field_f64<MESH>(f, x, t, B);
+    field_f64<FK>(f, x, t, B);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         dBdt[i] = 0.0;
@@ -532,9 +549,10 @@ __device__ __forceinline__ void field_jac_f64(const DevField& f, const double x[
         return;
     }
     case kFieldGrid:
-        grid_jac_f64(f, f.data, x, J);
+        if constexpr (FK >= kFkGrid) grid_jac_f64(f, f.data, x, J);
         return;
     case kFieldTimeGrid: {
+        if constexpr (FK < kFkGrid) return;
         // jacobian and derivative_t of LazyTimeInterpolator (interpolation.jl:396-430)
         double w;
         const int k = time_bracket(f, t, w);
@@ -557,7 +575,7 @@ __device__ __forceinline__ void field_jac_f64(const DevField& f, const double x[
         return;
     }
     case kFieldMesh:
-        if constexpr (MESH) {
+        if constexpr (FK == kFkMesh) {
             const MeshValJac r = mesh_value_jac(f.mesh, x[0], x[1], x[2]);
 #pragma unroll
             for (int a = 0; a < 3; ++a)
@@ -820,11 +838,11 @@ __device__ __forceinline__ void pos_abs(const KParams& p, const Pos32& x, double
                ((double)x.hi[d] + (double)x.lo[d]);
 }
 
-template <bool MESH = false>
+template <int FK = kFkMesh>
 __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, const Pos32& x,
                                           double t, float o[3])
 {
-    if constexpr (MESH) {
+    if constexpr (FK == kFkMesh) {
         if (f.kind == kFieldMesh) {
             // meshes are evaluated in FP64 at the absolute position
             double xa[3];
@@ -839,10 +857,18 @@ __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, c
         o[0] = (float)f.v[0]; o[1] = (float)f.v[1]; o[2] = (float)f.v[2];
         return;
     case kFieldGrid:
-        if (f.dtype == 1) grid_f32_t<float>(f, x, o);
-        else grid_f32_t<double>(f, x, o);
+        if constexpr (FK >= kFkGrid) {
+            if (f.dtype == 1) grid_f32_t<float>(f, x, o);
+            else grid_f32_t<double>(f, x, o);
+        } else {
+            o[0] = o[1] = o[2] = 0.0f;
+        }
         return;
     case kFieldTimeGrid: {
+        if constexpr (FK < kFkGrid) {
+            o[0] = o[1] = o[2] = 0.0f;
+            return;
+        }
         double w;
         const int k = time_bracket(f, t, w);
         const int k0 = k == 0 ? 0 : k - 1, k1 = k >= f.nsnap ? f.nsnap - 1 : k;

@@ -1,0 +1,714 @@
+// tpamd_pusher.h -- the solve kernels: precision policies, the fixed-step
+// and adaptive loops, the specialised gridded-B policies, the batched
+// chunk-queue kernel and the launchers.  Each kernel family's translation
+// unit includes this header and instantiates only its own kernels.
+//
+// One thread per particle; the whole integration (all nt steps, every save)
+// runs inside one launch, with the particle state in registers.  This
+// replaces the reference's accelerator path, which launches one kernel per
+// step, synchronises, and copies the full 6 x N state to the host for every
+// saved record (src/boris/boris_kernel.jl:130-236).
+#pragma once
+
+#include "tpamd_device.h"
+#include "tpamd_grid.h"
+#include "tpamd_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+using namespace tpamd;
+
+namespace {
+
+
+// ---------------------------------------------------------------------------
+// Precision policies
+// ---------------------------------------------------------------------------
+template <int FK>
+struct S64T {
+    static constexpr int kNS = -1;   // generic: any algorithm, adaptive or not
+    static constexpr int kFK = FK;   // field-kind class (kFk*)
+    using R = double;
+    struct Pos { double x[3]; };
+    Pos pos;
+    double v[3];
+    double dt_cache;
+
+    __device__ __forceinline__ void load(const KParams& p, long long i)
+    {
+        const double* s = p.xv0 + 6 * i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { pos.x[c] = s[c]; v[c] = s[3 + c]; }
+    }
+    __device__ __forceinline__ void set_dt(double dt) { dt_cache = dt; }
+    __device__ __forceinline__ void fields(const KParams& p, double t, double E[3], double B[3]) const
+    {
+        field_f64<FK>(p.B, pos.x, t, B);
+        if (p.e_is_uxb) cross3(B, p.E.v, E);       // E = -u x B = B x u
+        else field_f64<FK>(p.E, pos.x, t, E);
+    }
+    // fields at the current position, then the velocity update from given
+    // fields: update_velocity (boris.jl:38-45) / MultistepUpdater
+    // (multistep_boris.jl:7-9) is U = eval + Uf
+    __device__ __forceinline__ void eval(const KParams& p, double t, double E[3], double B[3]) const
+    {
+        fields(p, t, E, B);
+    }
+    template <int ALG>
+    __device__ __forceinline__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
+                       double h, double out[3]) const
+    {
+        if (ALG == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
+        else multistep_f64(p, vin, E, B, h, out);
+    }
+    template <int ALG>
+    __device__ __forceinline__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
+    {
+        double E[3], B[3];
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
+    }
+    __device__ __forceinline__ double bmag_of(const double B[3]) const { return sqrt(dot3(B, B)); }
+    __device__ __forceinline__ void step(const double vv[3], double dt, Pos& xn) const
+    {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xn.x[c] = pos.x[c] + vv[c] * dt;
+    }
+    __device__ __forceinline__ void abs_pos(const KParams&, const Pos& q, double o[3]) const
+    {
+        o[0] = q.x[0]; o[1] = q.x[1]; o[2] = q.x[2];
+    }
+    __device__ __forceinline__ void vel_d(const double vv[3], double o[3]) const
+    {
+        o[0] = vv[0]; o[1] = vv[1]; o[2] = vv[2];
+    }
+};
+
+template <int FK>
+struct S32CT {
+    static constexpr int kNS = -1;
+    static constexpr int kFK = FK;
+    using R = float;
+    using Pos = Pos32;
+    Pos pos;
+    float v[3];
+    float dt_f, dt_hi, dt_lo;
+
+    __device__ __forceinline__ void load(const KParams& p, long long i) { load_from(p, p.xv0 + 6 * i); }
+    __device__ __forceinline__ void load_from(const KParams& p, const double* s)
+    {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double r = s[c] - p.frame_origin[c];
+            int w = 0;
+            if (p.frame_period[c] > 0.0) {
+                const double q = floor(r / p.frame_period[c]);
+                w = (int)q;
+                r = r - q * p.frame_period[c];
+            }
+            pos.hi[c] = (float)r;
+            pos.lo[c] = (float)(r - (double)pos.hi[c]);
+            pos.wrap[c] = w;
+            v[c] = (float)s[3 + c];
+        }
+    }
+    __device__ __forceinline__ void set_dt(double dt)
+    {
+        dt_hi = (float)dt;
+        dt_lo = (float)(dt - (double)dt_hi);
+        dt_f = dt_hi;
+    }
+    __device__ __forceinline__ void fields(const KParams& p, const Pos& q, double t, float E[3], float B[3]) const
+    {
+        field_f32<FK>(p, p.B, q, t, B);
+        if (p.e_is_uxb) {
+            const float u[3] = {(float)p.E.v[0], (float)p.E.v[1], (float)p.E.v[2]};
+            cross3f(B, u, E);
+        } else {
+            field_f32<FK>(p, p.E, q, t, E);
+        }
+    }
+    __device__ __forceinline__ void eval(const KParams& p, double t, float E[3], float B[3]) const
+    {
+        fields(p, pos, t, E, B);
+    }
+    template <int ALG>
+    __device__ __forceinline__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
+                       double h, float out[3]) const
+    {
+        if (ALG == 0) boris_f32(vin, E, B, (float)((p.q2m * 0.5) * h), out);
+        else multistep_f32(p, vin, E, B, (float)h, out);
+    }
+    template <int ALG>
+    __device__ __forceinline__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
+    {
+        float E[3], B[3];
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
+    }
+    __device__ __forceinline__ double bmag_of(const float B[3]) const { return (double)sqrtf(dot3f(B, B)); }
+    __device__ __forceinline__ void step_impl(const KParams& p, const float vv[3], Pos& xn) const
+    {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float d = fmaf(vv[c], dt_hi, vv[c] * dt_lo);
+            float s, e;
+            two_sum(pos.hi[c], d + pos.lo[c], s, e);
+            int w = pos.wrap[c];
+            if (p.frame_periodic) {
+                const float P = (float)p.frame_period[c];
+                if (P > 0.0f && (s >= P || s < 0.0f)) {
+                    const float Plo = (float)(p.frame_period[c] - (double)P);
+                    const float sgn = s >= P ? 1.0f : -1.0f;
+                    float s2, e2;
+                    two_sum(s, -sgn * P, s2, e2);
+                    two_sum(s2, fmaf(-sgn, Plo, e + e2), s, e);
+                    w += (int)sgn;
+                }
+            }
+            xn.hi[c] = s;
+            xn.lo[c] = e;
+            xn.wrap[c] = w;
+        }
+    }
+    __device__ __forceinline__ void abs_pos(const KParams& p, const Pos& q, double o[3]) const { pos_abs(p, q, o); }
+    __device__ __forceinline__ void vel_d(const float vv[3], double o[3]) const
+    {
+        o[0] = (double)vv[0]; o[1] = (double)vv[1]; o[2] = (double)vv[2];
+    }
+};
+
+// generic policies per field-kind class: analytic fields, + grids, + meshes
+using S64A = S64T<kFkAnalytic>;
+using S64 = S64T<kFkGrid>;
+using S64M = S64T<kFkMesh>;
+using S32CA = S32CT<kFkAnalytic>;
+using S32C = S32CT<kFkGrid>;
+using S32CM = S32CT<kFkMesh>;
+
+template <class S>
+__device__ __forceinline__ bool outside(const KParams& p, const S& s, const typename S::Pos& xn,
+                                        const typename S::R vv[3])
+{
+    const int f = p.bnd_flags;
+    if (!f) return false;
+    double x[3], v[3];
+    s.abs_pos(p, xn, x);
+    s.vel_d(vv, v);
+    if (f & kBndNaN) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (x[c] != x[c] || v[c] != v[c]) return true;
+    }
+    if (f & (kBndSphereIn | kBndSphereOut)) {
+        // r = sqrt(|x|^2) compared with the radius; away from the sphere the
+        // squared norm alone decides it, near it the exact test runs
+        const double d = dot3(x, x);
+        if (f & kBndSphereIn) {
+            if (d < p.rin2_lo) return true;
+            if (d <= p.rin2_hi && sqrt(d) < p.r_in) return true;
+        }
+        if (f & kBndSphereOut) {
+            if (d > p.rout2_hi) return true;
+            if (d >= p.rout2_lo && sqrt(d) > p.r_out) return true;
+        }
+    }
+    if (f & kBndBox) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (x[c] < p.box_lo[c] || x[c] > p.box_hi[c]) return true;
+    }
+    return false;
+}
+
+// extra record columns (E, B and/or work rates) at (x, v, t), in FP64.  Only
+// the REC instantiations of the generic kernels contain it, so the pushers
+// that save plain records keep their register budget.
+template <int FK>
+__device__ __forceinline__ void record_extra(const KParams& p, const double x[3], const double v[3],
+                                          double t, double* r)
+{
+    double E[3], B[3], J[3][3], dBdt[3];
+    field_jac_f64<FK>(p.B, x, t, B, J, dBdt);
+    if (p.e_is_uxb) cross3(B, p.E.v, E);                  // E = -u x B = B x u
+    else field_f64<FK>(p.E, x, t, E);
+    int c = 6;
+    if (p.save_fields) {
+        r[6] = E[0]; r[7] = E[1]; r[8] = E[2];
+        r[9] = B[0]; r[10] = B[1]; r[11] = B[2];
+        c = 12;
+    }
+    if (p.save_work) {
+        double w[4];
+        work_rates_f64(p.q2m, p.mass, v, E, B, J, dBdt, w);
+        r[c] = w[0]; r[c + 1] = w[1]; r[c + 2] = w[2]; r[c + 3] = w[3];
+    }
+}
+
+// one record (_prepare_saved_data, boris.jl:405-440): x, v, then E and B,
+// then the work rates, as the descriptor asks (REC kernels only)
+template <bool REC, class S>
+__device__ __forceinline__ void record(const KParams& p, const S& s, long long i, long long k,
+                                       const typename S::R vv[3], double t)
+{
+    double x[3], v[3];
+    s.abs_pos(p, s.pos, x);
+    s.vel_d(vv, v);
+    double* r = p.out_u + (i * p.cap + k) * p.rec_w;
+    r[0] = x[0]; r[1] = x[1]; r[2] = x[2];
+    r[3] = v[0]; r[4] = v[1]; r[5] = v[2];
+    if constexpr (REC) record_extra<S::kFK>(p, x, v, t, r);
+    p.out_t[i * p.cap + k] = t;
+}
+
+template <class S>
+__device__ __forceinline__ void do_step(const KParams& p, const S& s, const typename S::R vv[3],
+                                        double dt, typename S::Pos& xn)
+{
+    if constexpr (sizeof(typename S::R) == 8) s.step(vv, dt, xn);
+    else s.step_impl(p, vv, xn);
+}
+
+// _boris_single + _boris_loop!, src/boris/boris.jl:445-557.  The save test
+// `(it - 1) % K == 0` is carried as a countdown to the next save step.
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
+{
+    using R = typename S::R;
+    S s;
+    s.load(p, i);
+    const double dt = p.dt_pp ? p.dt_pp[i] : p.dt;
+    const double t0 = p.t0;
+    const double t1 = p.dt_pp ? t0 + (double)p.nt * dt : p.t1;
+    s.set_dt(dt);
+    long long iout = 0;
+    double tlast = 0.0;
+    if (p.save_start) {                                       // :507-511
+        record<REC>(p, s, i, iout, s.v, t0);
+        tlast = t0;
+        iout += 1;
+    }
+    R vt[3];
+    s.template U<ALG>(p, s.v, -0.5 * dt, t0, vt);             // :514
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+
+    const long long nt = p.nt;
+    long long it = 1;
+    long long next_save = p.save_every ? p.K + 1 : nt + 2;    // first it with (it-1) % K == 0, it-1 > 0
+    double t = fma(-0.5, dt, t0);                             // :451
+    for (; it <= nt; ++it) {
+        R vprev[3] = {s.v[0], s.v[1], s.v[2]};
+        t += dt;                                              // :454
+        R Ec[3], Bc[3];
+        s.eval(p, t, Ec, Bc);
+        s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vt);           // :455
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+        typename S::Pos xn;
+        do_step(p, s, s.v, dt, xn);                           // :457
+        if (outside(p, s, xn, s.v)) {                         // :459-461
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s.v[c] = vprev[c];
+            break;
+        }
+        if (it == next_save) {                                // :463-472
+            next_save += p.K;
+            iout += 1;
+            if (iout <= p.nout) {
+                const double tc = fma(-0.5, dt, t);
+                R vs[3];
+                // same position as the step's evaluation: static fields are re-used
+                if (p.static_fields) s.template Uf<ALG>(p, Ec, Bc, vprev, 0.5 * dt, vs);
+                else s.template U<ALG>(p, vprev, 0.5 * dt, tc, vs);
+                record<REC>(p, s, i, iout - 1, vs, tc);
+                tlast = tc;
+            }
+        }
+        s.pos = xn;                                           // :474
+    }
+    const long long final_step = it - 1;
+    bool should = p.save_end ||
+                  (p.save_every && final_step > 0 && final_step % p.K == 0);     // :522-528
+    if (iout < p.nout && should) {                                            // :530-543
+        const double tf = (final_step == nt) ? t1 : fma((double)final_step, dt, t0);
+        if (iout == 0 || tlast < tf) {
+            const double dtf = tf - fma((double)final_step - 0.5, dt, t0);
+            R vf[3];
+            s.template U<ALG>(p, s.v, dtf, tf, vf);
+            record<REC>(p, s, i, iout, vf, tf);
+            iout += 1;
+        }
+    }
+    p.out_len[i] = (int)iout;                                                 // :545-551
+    p.retcode[i] = (iout < p.nout) ? kRetTerminated : kRetSuccess;
+    return final_step;
+}
+
+// _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
+{
+    using R = typename S::R;
+    S s;
+    s.load(p, i);
+    const double t0 = p.t0, t1 = p.t1;
+    const double C = p.C;                                     // :164
+    const double ttotal = t1 - t0;
+    double t = t0;
+    long long nrec = 0;
+    double tlast = 0.0;
+    R vt[3];
+    auto push = [&](const R vv[3], double tt) {
+        if (nrec < p.cap) record<REC>(p, s, i, nrec, vv, tt);
+        tlast = tt;
+        nrec += 1;
+    };
+    if (p.save_start) push(s.v, t);                           // :179-186
+    // Ec, Bc: the fields at the current (x, t).  The reference evaluates
+    // there up to four times per step (|B| for dt, both halves of the
+    // resynchronisation, a save); one evaluation serves them all, and with
+    // static fields the mid-step update as well.
+    R Ec[3], Bc[3];
+    s.eval(p, t, Ec, Bc);
+    double dt = C / s.bmag_of(Bc);                            // :188-189
+    s.set_dt(dt);
+    s.template Uf<ALG>(p, Ec, Bc, s.v, -0.5 * dt, vt);        // :192
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+    long long it = 1;
+    long long next_save = p.save_every ? p.K + 1 : -1;
+    bool should = p.save_end;
+    int ret = kRetSuccess;
+    // every wave must leave the loop: a zero step (|B| = inf) or an
+    // unbounded run stops with MaxIters instead of spinning
+    const long long cap_steps = p.max_steps > 0 ? p.max_steps : kAdaptiveHardCap;
+    while (fabs(t - t0) < fabs(ttotal)) {                     // :197
+        if (dt == 0.0) {
+            ret = kRetMaxIters;
+            break;
+        }
+        if (fabs(t + dt - t0) > fabs(ttotal)) {               // :198-203
+            const double dts = t1 - t;
+            s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
+            s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dts, s.v);
+            dt = dts;
+            s.set_dt(dt);
+        }
+        if (it == next_save) {                                // :205-218
+            next_save += p.K;
+            s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
+            push(vt, t);
+        }
+        const double tmid = t + 0.5 * dt;                     // :220
+        R vn[3];
+        if (p.static_fields) s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vn);   // :221
+        else s.template U<ALG>(p, s.v, dt, tmid, vn);
+        typename S::Pos xn;
+        do_step(p, s, vn, dt, xn);                            // :223
+        const double tn = t + dt;
+        if (outside(p, s, xn, vn)) {                          // :227-231
+            should = true;
+            ret = kRetTerminated;
+            break;
+        }
+        s.pos = xn;                                           // :233-235
+        t = tn;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.v[c] = vn[c];
+        s.eval(p, t, Ec, Bc);
+        const double dtn = C / s.bmag_of(Bc);                 // :237-238
+        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);     // :241-242
+        s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dtn, s.v);
+        dt = dtn;
+        s.set_dt(dt);
+        it += 1;
+        if (it == next_save) should = true;                   // :245-247, (it-1) % K == 0
+        if (it - 1 >= cap_steps) {
+            ret = kRetMaxIters;
+            break;
+        }
+    }
+    if (should && (nrec == 0 || tlast != t)) {                // :250-259
+        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
+        push(vt, t);
+    }
+    p.out_len[i] = (int)nrec;
+    p.retcode[i] = ret;
+    return it - 1;
+}
+
+__device__ __forceinline__ void add_steps(const KParams& p, long long steps)
+{
+    // wave-level sum, then one atomic per wave
+    unsigned long long v = (unsigned long long)steps;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(p.steps, v);
+}
+
+template <class S, int ALG, bool REC = false>
+__global__ void __launch_bounds__(256) k_fixed(const KParams p)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long steps = 0;
+    if (i < p.N) steps = run_fixed<S, ALG, REC>(p, i);
+    add_steps(p, steps);
+}
+
+template <class S, int ALG, bool REC = false>
+__global__ void __launch_bounds__(256) k_adaptive(const KParams p)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long steps = 0;
+    if (i < p.N) steps = run_adaptive<S, ALG, REC>(p, i);
+    add_steps(p, steps);
+}
+
+// ---------------------------------------------------------------------------
+// Specialised gridded-B policies (see tpamd_grid.h).  NS = 0 is the Boris
+// update, NS >= 1 the rational multistep update with n = NS subcycles.
+// ---------------------------------------------------------------------------
+template <int EMODE, int NS, bool PER, typename T, bool CELLS>
+struct S64G : S64 {
+    static constexpr int kNS = NS;
+    __device__ __forceinline__ void bfield(const KParams& p, double B[3]) const
+    {
+        int i0[3], i1[3];
+        double w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double s = (pos.x[d] - p.B.origin[d]) * p.B.inv_h[d];
+            if (!locate_f64<PER>(p.B, d, s, i0[d], i1[d], w[d])) {
+                B[0] = B[1] = B[2] = __builtin_nan("");
+                return;
+            }
+        }
+        if constexpr (CELLS) {
+            trilinear_cell<T, double>(p.B.cells, cell_offset(p.B, i0), w, B);
+        } else {
+            unsigned ox[2], oy[2], oz[2];
+            offsets(p.B, i0, i1, sizeof(T), ox, oy, oz);
+            trilinear<T, double>(p.B.data, ox, oy, oz, w, B);
+        }
+    }
+    __device__ __forceinline__ void eval(const KParams& p, double, double E[3], double B[3]) const
+    {
+        bfield(p, B);
+        if (EMODE == kEUxB) cross3(B, p.E.v, E);
+        else { E[0] = p.E.v[0]; E[1] = p.E.v[1]; E[2] = p.E.v[2]; }
+    }
+    template <int ALG>
+    __device__ __forceinline__ void Uf(const KParams& p, const double E[3], const double B[3], const double vin[3],
+                       double h, double out[3]) const
+    {
+        if (NS == 0) boris_f64(vin, E, B, (p.q2m * 0.5) * h, out);
+        else multistep_f64_n<NS == 0 ? 1 : NS>(p, vin, E, B, h, out);
+    }
+    template <int ALG>
+    __device__ __forceinline__ void U(const KParams& p, const double vin[3], double h, double t, double out[3]) const
+    {
+        double E[3], B[3];
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
+    }
+};
+
+template <int EMODE, int NS, bool PER, bool CELLS>
+struct S32G : S32C {
+    static constexpr int kNS = NS;
+    __device__ __forceinline__ void bfield(const KParams& p, float B[3]) const
+    {
+        int i0[3], i1[3];
+        float w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float s = fmaf(pos.hi[d], p.B.inv_hf[d], pos.lo[d] * p.B.inv_hf[d]);
+            if (!locate_f32<PER>(p.B, d, s, i0[d], i1[d], w[d])) {
+                B[0] = B[1] = B[2] = __builtin_nanf("");
+                return;
+            }
+        }
+        if constexpr (CELLS) {
+            trilinear_cell<float, float>(p.B.cells, cell_offset(p.B, i0), w, B);
+        } else {
+            unsigned ox[2], oy[2], oz[2];
+            offsets(p.B, i0, i1, 4u, ox, oy, oz);
+            trilinear<float, float>(p.B.data, ox, oy, oz, w, B);
+        }
+    }
+    __device__ __forceinline__ void eval(const KParams& p, double, float E[3], float B[3]) const
+    {
+        bfield(p, B);
+        if (EMODE == kEUxB) cross3f(B, p.f32.u, E);
+        else { E[0] = p.f32.e0[0]; E[1] = p.f32.e0[1]; E[2] = p.f32.e0[2]; }
+    }
+    template <int ALG>
+    __device__ __forceinline__ void Uf(const KParams& p, const float E[3], const float B[3], const float vin[3],
+                       double h, float out[3]) const
+    {
+        if (NS == 0) boris_f32_fast(vin, E, B, (float)((p.q2m * 0.5) * h), out);
+        else multistep_f32_n<NS == 0 ? 1 : NS>(p, vin, E, B, (float)(p.q2m_2n * h), out);
+    }
+    template <int ALG>
+    __device__ __forceinline__ void U(const KParams& p, const float vin[3], double h, double t, float out[3]) const
+    {
+        float E[3], B[3];
+        eval(p, t, E, B);
+        Uf<ALG>(p, E, B, vin, h, out);
+    }
+};
+
+template <class S, int ALG, bool ADAPT, bool REC = false>
+__global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table,
+                                               const Chunk* __restrict__ chunks, int nchunks,
+                                               unsigned* __restrict__ counter,
+                                               unsigned long long* __restrict__ steps_out)
+{
+    const int lane = threadIdx.x & 63;
+    long long steps = 0;
+    // the chunk index is made wave-uniform (readfirstlane), so the chunk and
+    // the instance's parameter block are read with scalar loads; the next
+    // chunk is claimed before the current one is worked on, which hides the
+    // atomic's round trip
+    unsigned claim = 0;
+    if (lane == 0) claim = atomicAdd(counter, 1u);
+    unsigned c = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+    while (c < (unsigned)nchunks) {
+        unsigned next = 0;
+        if (lane == 0) next = atomicAdd(counter, 1u);
+        const Chunk ch = chunks[c];
+        const KParams& p = table[ch.inst];
+        for (long long i = ch.begin + lane; i < ch.end; i += 64) {
+            if constexpr (ADAPT) steps += run_adaptive<S, ALG, REC>(p, i);
+            else steps += run_fixed<S, ALG, REC>(p, i);
+        }
+        c = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
+    }
+    unsigned long long v = (unsigned long long)steps;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) atomicAdd(steps_out, v);
+}
+
+// ---------------------------------------------------------------------------
+// Policy selection.  select_policy calls l.go<S, ALG>() with the specialised
+// gridded-B policy that matches the solve, or with the generic policy.  The
+// specialisations cover: B a grid whose bytes fit 32-bit offsets, E zero,
+// uniform or UxB, the Boris update (fixed or adaptive) or the rational
+// multistep update with n <= 2 (fixed step), FP32C on float storage.
+// ---------------------------------------------------------------------------
+template <class L, int EMODE, int NS, bool CELLS>
+void sel_g32(const L& l, bool per)
+{
+    if (per) l.template go<S32G<EMODE, NS, true, CELLS>, NS == 0 ? 0 : 1>();
+    else l.template go<S32G<EMODE, NS, false, CELLS>, NS == 0 ? 0 : 1>();
+}
+
+template <class L, int EMODE, int NS, typename T, bool CELLS>
+void sel_g64(const L& l, bool per)
+{
+    if (per) l.template go<S64G<EMODE, NS, true, T, CELLS>, NS == 0 ? 0 : 1>();
+    else l.template go<S64G<EMODE, NS, false, T, CELLS>, NS == 0 ? 0 : 1>();
+}
+
+template <class L, int EMODE, bool CELLS>
+void sel_g32_ns(const L& l, int ns, bool per)
+{
+    if (ns == 0) sel_g32<L, EMODE, 0, CELLS>(l, per);
+    else if (ns == 1) sel_g32<L, EMODE, 1, CELLS>(l, per);
+    else sel_g32<L, EMODE, 2, CELLS>(l, per);
+}
+
+template <class L, int EMODE, typename T, bool CELLS>
+void sel_g64_ns(const L& l, int ns, bool per)
+{
+    if (ns == 0) sel_g64<L, EMODE, 0, T, CELLS>(l, per);
+    else if (ns == 1) sel_g64<L, EMODE, 1, T, CELLS>(l, per);
+    else sel_g64<L, EMODE, 2, T, CELLS>(l, per);
+}
+
+// one solve: a thread per particle
+struct SoloLaunch {
+    const KParams& p;
+    int adaptive;
+    dim3 g, b;
+    hipStream_t st;
+    template <class S, int ALG>
+    void go() const
+    {
+        // records with E, B or work columns: generic policies only (spec_class
+        // declines them), in the REC instantiations
+        if constexpr (S::kNS < 0) {
+            if (p.rec_w > 6) {
+                if (adaptive) hipLaunchKernelGGL((k_adaptive<S, ALG, true>), g, b, 0, st, p);
+                else hipLaunchKernelGGL((k_fixed<S, ALG, true>), g, b, 0, st, p);
+                return;
+            }
+        }
+        if constexpr (S::kNS <= 0) {
+            if (adaptive) {
+                hipLaunchKernelGGL((k_adaptive<S, ALG>), g, b, 0, st, p);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_fixed<S, ALG>), g, b, 0, st, p);
+    }
+};
+
+// batched solve: persistent waves over the chunk queue.  With `occ` set it
+// only reports the kernel's resident blocks per CU for block size b.x.
+struct BatchLaunch {
+    const KParams* table;
+    const Chunk* chunks;
+    int nchunks;
+    unsigned* counter;
+    unsigned long long* steps;
+    int adaptive;
+    dim3 g, b;
+    hipStream_t st;
+    int* occ;
+    bool rec;                 // records carry E, B or work columns
+    template <class K>
+    void run(K kern) const
+    {
+        if (occ) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, kern, (int)b.x, 0) != hipSuccess)
+                *occ = 1;
+            return;
+        }
+        hipLaunchKernelGGL(kern, g, b, 0, st, table, chunks, nchunks, counter, steps);
+    }
+    template <class S, int ALG>
+    void go() const
+    {
+        if constexpr (S::kNS < 0) {
+            if (rec) {
+                if (adaptive) run(k_batch<S, ALG, true, true>);
+                else run(k_batch<S, ALG, false, true>);
+                return;
+            }
+        }
+        if constexpr (S::kNS <= 0) {
+            if (adaptive) {
+                run(k_batch<S, ALG, true>);
+                return;
+            }
+        }
+        run(k_batch<S, ALG, false>);
+    }
+};
+
+template <class F>
+void with_launcher(const LaunchReq& r, const F& f)
+{
+    if (r.batch)
+        f(BatchLaunch{r.table, r.chunks, r.nchunks, r.counter, r.steps, r.adaptive, r.g, r.b, r.st,
+                      r.occ, r.rec});
+    else
+        f(SoloLaunch{*r.p, r.adaptive, r.g, r.b, r.st});
+}
+
+}  // namespace
