@@ -25,5 +25,6 @@ from .solution import EnsembleSolution, TraceSolution
 from .solve import initial_states, make_desc, plan, solve, solve_batch
 from .adjoint import AdjointResult, loss_gradient, trajectory_loss
 from .postprocess import get_fields, get_work, record_eval
+from .wrapper import TraceResult, trace, trace_fieldline, trace_gc
 
 __all__ = [n for n in dir() if not n.startswith("_")]
