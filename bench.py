@@ -47,6 +47,7 @@ GATHER_BYTES = {"f32": 8 * 3 * 4, "f64": 8 * 3 * 8}
 # algorithmic FLOPs per particle-step (FMA = 2; div, sqrt = 1): standard Boris
 # with E = 0 (37) + position update (6) + dipole evaluation (27)
 DIPOLE_FLOP = 37 + 6 + 27
+QI, MI = 1.602176634e-19, 1.67262192369e-27   # proton charge and mass (constants.jl:4-20)
 SECTIONS = ("headline", "dipole", "batch", "adjoint")
 
 
@@ -102,10 +103,20 @@ def cpu_baseline(w, A64, ax, nthreads, target_s=12.0):
     rate = probe * w.nsteps / max(r.elapsed, 1e-6)
     n = int(min(len(w.xv0), max(probe, rate * target_s / w.nsteps)))
     r = O.solve(w.xv0[:n], E, B, **kw)
+    # calibration: one thread on the reference's serial benchmark problem
+    # (benchmark/bench_serial.jl:10-45: proton, uniform B = (0, 0, 1e-8) T,
+    # E = 0, v0 = (1e5, 0, 0), dt = 1 ns, 1e8 steps, a record every 1e5 steps),
+    # published at 1.08e8 particle-steps/s per core (docs/src/index.md:48-54)
+    cal = O.solve(np.array([[0.0, 0.0, 0.0, 1.0e5, 0.0, 0.0]]), O.zero(), O.uniform([0.0, 0.0, 1.0e-8]),
+                  q2m=QI / MI, tspan=(0.0, 0.1), dt=1.0e-9, savestepinterval=100000,
+                  maxiters=200_000_000, nthreads=1)
     return {"value": n * w.nsteps / r.elapsed, "unit": "particle-steps/s", "cores": nthreads,
             "kind": "port",
             "sample": f"{n} of {len(w.xv0)} particles x {w.nsteps} steps, FP64 "
-                      f"MultistepBoris4(n=2) reference form, {r.elapsed:.1f} s wall"}
+                      f"MultistepBoris4(n=2) reference form, {r.elapsed:.1f} s wall",
+            "calibration": {"problem": "bench_serial.jl: 1 proton, uniform B, 1e8 Boris steps",
+                            "threads": 1, "value": 1.0e8 / cal.elapsed,
+                            "published_per_core": 1.08e8}}
 
 
 # ---------------------------------------------------------------------------

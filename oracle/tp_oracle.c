@@ -41,9 +41,11 @@ static inline double dot3(const double a[3], const double b[3])
 }
 static inline double sumsq3(const double a[3]) { return dot3(a, a); }
 
-/* boris_velocity_update, src/boris/boris.jl:19-31 */
-void orc_boris_velocity_update(const double v[3], const double E[3],
-                               const double B[3], double h, double out[3])
+/* boris_velocity_update, src/boris/boris.jl:19-31 (inlined into the solve
+ * loops; orc_boris_velocity_update is the exported entry) */
+static inline __attribute__((always_inline)) void boris_update(const double v[3], const double E[3],
+                                                               const double B[3], double h,
+                                                               double out[3])
 {
     double t[3], s[3], vm[3], vmt[3], vp[3], vps[3];
     for (int c = 0; c < 3; ++c) t[c] = h * B[c];                /* :20 */
@@ -56,6 +58,12 @@ void orc_boris_velocity_update(const double v[3], const double E[3],
     cross3(vp, s, vps);
     for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vps[c];          /* :26 */
     for (int c = 0; c < 3; ++c) out[c] = vp[c] + h * E[c];       /* :28 */
+}
+
+void orc_boris_velocity_update(const double v[3], const double E[3],
+                               const double B[3], double h, double out[3])
+{
+    boris_update(v, E, B, h, out);
 }
 
 /* ----------------------------------------------------------------------------
@@ -867,16 +875,29 @@ typedef struct {
     const orc_field* B;
 } ctx_t;
 
+/* orc_field_eval with the constant kinds decided inline */
+static inline __attribute__((always_inline)) void field_eval_fast(const orc_field* f, const double x[3],
+                                                                  double t, double out[3])
+{
+    if (f->kind == ORC_FIELD_ZERO) {
+        out[0] = out[1] = out[2] = 0.0;
+    } else if (f->kind == ORC_FIELD_UNIFORM) {
+        out[0] = f->v[0]; out[1] = f->v[1]; out[2] = f->v[2];
+    } else {
+        orc_field_eval(f, x, t, out);
+    }
+}
+
 /* update_velocity (boris.jl:38-45) or MultistepUpdater (multistep_boris.jl:3-9) */
-static void updater(const ctx_t* c, const double v[3], const double x[3], double h,
+static inline __attribute__((always_inline)) void updater(const ctx_t* c, const double v[3], const double x[3], double h,
                     double t, double out[3])
 {
     double E[3], B[3];
-    orc_field_eval(c->E, x, t, E);                       /* boris.jl:40 */
-    orc_field_eval(c->B, x, t, B);                       /* boris.jl:41 */
+    field_eval_fast(c->E, x, t, E);                      /* boris.jl:40 */
+    field_eval_fast(c->B, x, t, B);                      /* boris.jl:41 */
     if (c->d->alg == ORC_ALG_BORIS) {
         const double h2 = (c->d->q2m * 0.5) * h;         /* boris.jl:42 */
-        orc_boris_velocity_update(v, E, B, h2, out);
+        boris_update(v, E, B, h2, out);
     } else {
         orc_multistep_velocity(v, E, B, c->d->q2m, h, c->d->nsub, c->d->order_N,
                                c->d->form, out);
@@ -885,7 +906,8 @@ static void updater(const ctx_t* c, const double v[3], const double x[3], double
 
 /* isoutside(u, p, t) menu; the reference passes an arbitrary predicate
  * (boris.jl:95, :459; adaptive_boris.jl:168, :369). */
-static int outside(const orc_desc* d, const double x[3], const double v[3])
+static inline __attribute__((always_inline)) int outside(const orc_desc* d, const double x[3],
+                                                          const double v[3])
 {
     const int f = d->bnd_flags;
     if (!f) return 0;
@@ -982,6 +1004,7 @@ static void solve_fixed_one(const ctx_t* c, long i, const double* xv0, long nt, 
     long it = 1;
     long final_step = nt;
     double t = fma(-0.5, dt, t0);                               /* :451 @muladd scalar */
+    long phase = 0;                    /* (it - 1) % K, kept as a counter */
     while (it <= nt) {
         double vprev[3] = {v[0], v[1], v[2]};
         t += dt;                                                /* :454 */
@@ -994,7 +1017,7 @@ static void solve_fixed_one(const ctx_t* c, long i, const double* xv0, long nt, 
             memcpy(v, vprev, sizeof v);
             goto loop_done;
         }
-        if (d->save_everystep && (it - 1) > 0 && (it - 1) % K == 0) {   /* :463 */
+        if (d->save_everystep && (it - 1) > 0 && phase == 0) {   /* :463, (it - 1) % K == 0 */
             iout += 1;
             if (iout <= nout) {
                 const double tc = fma(-0.5, dt, t);             /* :466 @muladd scalar */
@@ -1005,6 +1028,7 @@ static void solve_fixed_one(const ctx_t* c, long i, const double* xv0, long nt, 
         }
         memcpy(x, xn, sizeof x);                                /* :474 */
         it += 1;
+        if (++phase == K) phase = 0;
     }
     final_step = it - 1;
 loop_done:;
