@@ -1,12 +1,14 @@
 # TestParticleAMD.jl -- thin ccall layer from TestParticle.jl to libtpamd.so.
 #
-# Loaded as a package extension of TestParticle.jl, it adds one method of
+# Loaded as a package extension of TestParticle.jl, it adds a method of
 # `solve` that dispatches on the device backend `MI355X()`, mirroring the
 # KernelAbstractions entry point
 #     solve(prob, ::Boris{false}, backend::KA.Backend, ensemblealg; dt, ...)
 # (src/boris/boris_kernel.jl:312-382), and extends it to MultistepBoris and the
-# adaptive solvers.  Fields are described with the device menu (DeviceField)
-# rather than Julia closures, since a GPU cannot call host code.
+# adaptive solvers; and methods that route the reference's own
+# `solve(prob, Boris(); dt, ...)` to the GPU when p carries device fields.
+# Fields are described with the device menu (DeviceField) rather than Julia
+# closures, since a GPU cannot call host code.
 #
 # Untested here: the build container has no Julia.  The C ABI it calls is
 # include/tpamd.h, which the Python host tests exercise.
@@ -279,6 +281,52 @@ function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensembleal
 end
 
 # isoutside menu: (:nan, :sphere_in => R, :sphere_out => R, :box => (lo, hi))
+# --- preferred route: device fields in p -----------------------------------
+# When p = (q2m, m, E, B, F) carries device fields, the reference's own call
+#     TestParticle.solve(prob, Boris(); dt, savestepinterval)
+# lands on the GPU.  These methods mirror the reference's solve signatures
+# (src/boris/boris.jl:88-106, src/boris/adaptive_boris.jl:21-33) with the
+# same algorithm and ensemble types, and add a constraint on p that names
+# this module's field types: more specific than the reference's methods in
+# every argument, so no ambiguity and no type piracy.
+const DeviceParams = Tuple{Any, Any, DeviceField, DeviceField, Any}
+const FixedAlg = Union{Boris{false}, MultistepBoris{N, false} where {N}}
+const AdaptiveAlg = Union{Boris{true}, MultistepBoris{N, true} where {N}}
+
+function TestParticle.solve(prob::TraceProblem{U, T, I, P}, alg::FixedAlg,
+        ensemblealg::TestParticle.BasicEnsembleAlgorithm = TestParticle.EnsembleSerial();
+        device::Int = 0, kw...) where {U, T, I, P <: DeviceParams}
+    return TestParticle.solve(prob, alg, MI355X(device), ensemblealg; kw...)
+end
+function TestParticle.solve(prob::TraceProblem{U, T, I, P}, alg::AdaptiveAlg,
+        ensemblealg::TestParticle.BasicEnsembleAlgorithm = TestParticle.EnsembleSerial();
+        device::Int = 0, kw...) where {U, T, I, P <: DeviceParams}
+    return TestParticle.solve(prob, alg, MI355X(device), ensemblealg; kw...)
+end
+
+"""
+    prepare(x, y, z, E, B; device = MI355X(), order = 1, bc = 1, species, q, m, storage = Float64)
+    prepare(E, B; device = MI355X(), species, q, m)
+
+`TestParticle.prepare` for the device path: gridded arrays become
+`GridDeviceField`s (bc: 0 fill NaN, 1 clamp, 2 periodic), `DeviceField`s pass
+through, and the result routes `TestParticle.solve(prob, Boris(); ...)` to
+the GPU (src/prepare.jl:105-170).
+"""
+function prepare(x::AbstractRange, y::AbstractRange, z::AbstractRange, E, B;
+        device::MI355X = MI355X(), bc::Integer = 0, storage::Type = Float64, kw...)
+    mk(f) = f isa DeviceField ? f :
+        f === nothing ? ZeroDeviceField() :
+        GridDeviceField(Array(f), x, y, z, Int32(bc), storage === Float32 ? Int32(1) : Int32(0))
+    return prepare(mk(E), mk(B); device, kw...)
+end
+function prepare(E::DeviceField, B::DeviceField; device::MI355X = MI355X(),
+        species = TestParticle.Proton, q = nothing, m = nothing)
+    qq = q === nothing ? species.q : q
+    mm = m === nothing ? species.m : m
+    return (qq / mm, mm, E, B, TestParticle.ZeroField())
+end
+
 function apply_boundary!(d::SolveDesc, spec)
     for s in (spec isa Tuple ? spec : (spec,))
         if s === :nan
