@@ -320,9 +320,13 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     while ((1u << cellbits) < ncell) ++cellbits;
     const bool partition = cellbits > kShift;
     const unsigned nbins = (ncell + AdjBin::kCells - 1) / AdjBin::kCells;
+    // the block-local partition while its bin counters fit in LDS, else a radix sort
+    const bool local = partition && nbins <= AdjPart::kMaxBins;
+    const long long nblk = (M + AdjPart::part<R>() - 1) / AdjPart::part<R>();
+    const long long ncnt = local ? (long long)nbins * nblk : 0;
     size_t temp_bytes = 0;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    if (partition)
+    if (partition && !local)
         HIP_TRY(ctx, adj_sort_records<R>(nullptr, temp_bytes, nullptr, nullptr, nullptr, nullptr, M, kShift,
                                          cellbits, ctx->stream));
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -331,9 +335,10 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
                  b_st = al(rs * 6 * (size_t)N), b_key = al(sizeof(unsigned) * (size_t)M),
                  b_val = al(sizeof(AdjRec<R>) * (size_t)M), b_tmp = al(std::max<size_t>(temp_bytes, 1)),
                  b_G = al(sizeof(double) * 24 * (size_t)ncell), b_out = al(sizeof(double) * 3 * nnodes),
-                 b_gx = al(sizeof(double) * 6 * N), b_ls = al(sizeof(double)), b_xf = al(sizeof(double) * 3 * N);
+                 b_gx = al(sizeof(double) * 6 * N), b_ls = al(sizeof(double)), b_xf = al(sizeof(double) * 3 * N),
+                 b_cnt = al(sizeof(unsigned) * (size_t)std::max<long long>(ncnt, 1));
     const size_t total = b_xv + b_tg + b_ck + b_sg + b_st + 2 * b_key + 2 * b_val + b_tmp + b_G + b_out +
-                         b_gx + b_ls + b_xf;
+                         b_gx + b_ls + b_xf + 2 * b_cnt;
     char* buf = nullptr;
     HIP_TRY(ctx, hipMalloc((void**)&buf, total));
     char* q = buf;
@@ -354,6 +359,8 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     a.grad_xv0 = (double*)take(b_gx);
     a.loss = (double*)take(b_ls);
     a.xfinal = (double*)take(b_xf);
+    unsigned* d_cnt = (unsigned*)take(b_cnt);      // segment starts [bin][block]
+    unsigned* d_off = (unsigned*)take(b_cnt);      // segment counts [bin][block]
     a.targets = d_tg;
     a.rec_key = keys0;
     a.rec_val = vals0;
@@ -384,12 +391,17 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     for (int seg = nseg - 1; seg >= 0 && e == hipSuccess; --seg) {
         hipLaunchKernelGGL((k_adj_reverse<F32, PER, T, CELLS>), grid, block, 0, st, p, a, seg);
         e = hipGetLastError();
-        if (e == hipSuccess && partition)
-            e = adj_sort_records<R>(d_tmp, temp_bytes, keys0, keys1, vals0, vals1, M, kShift, cellbits, st);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL((k_adj_bins<R>), dim3(nbins), dim3(AdjBin::kThreads), 0, st,
-                               partition ? keys1 : keys0, partition ? vals1 : vals0, M, ncell, G);
-            e = hipGetLastError();
+        if (e == hipSuccess && local) {
+            e = adj_localsort_records<R>(keys0, vals0, M, nbins, keys1, vals1, d_cnt, d_off, st);
+            if (e == hipSuccess) e = adj_bins_seg<R>(keys1, vals1, d_cnt, d_off, (int)nblk, nbins, ncell, G, st);
+        } else if (e == hipSuccess) {
+            if (partition)
+                e = adj_sort_records<R>(d_tmp, temp_bytes, keys0, keys1, vals0, vals1, M, kShift, cellbits, st);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL((k_adj_bins<R>), dim3(nbins), dim3(AdjBin::kThreads), 0, st,
+                                   partition ? keys1 : keys0, partition ? vals1 : vals0, M, ncell, G);
+                e = hipGetLastError();
+            }
         }
     }
     if (e == hipSuccess) {

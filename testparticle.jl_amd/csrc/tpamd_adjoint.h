@@ -48,6 +48,22 @@ struct AdjBin {
     static constexpr int kThreads = 1024;
 };
 
+// Partition of the records by bin (key >> kShift), used while the bin
+// counters fit in LDS: each block of kPart consecutive records is sorted by
+// bin in LDS and written back in place (contiguous, coalesced stores), with
+// the count and block-local start of every (bin, block) segment in [bin][block]
+// tables; the bin kernel then reads its bin's segment from every block.  One
+// read and one write of the records, against two radix passes and a global
+// scatter.  Records keep their bin, not their order within a bin (the bin
+// sums are order-free LDS atomics already).
+struct AdjPart {
+    static constexpr int kThreads = 1024;
+    static constexpr unsigned kMaxBins = 2048;
+    // records per block: the block's records and keys staged in LDS
+    template <typename R>
+    static constexpr int part() { return sizeof(R) == 4 ? 4096 : 2048; }
+};
+
 struct AdjParams {
     const double* targets;    // [N][3]
     double* ckpt;             // [nseg][9][N]: frame position (3), wrap (3), velocity (3)
@@ -192,6 +208,23 @@ __device__ __forceinline__ void adj_xgrad(const DevField& f, const R c[8][3], co
     for (int d = 0; d < 3; ++d) xb[d] = g[d] * (R)f.inv_h[d];
 }
 
+// one record's corner moments into the bin's LDS tile (component-major)
+template <typename R>
+__device__ __forceinline__ void adj_tile_add(double* tile, unsigned local, const AdjRec<R>& v)
+{
+    constexpr int kCells = AdjBin::kCells;
+    double* t = tile + local;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const R W = ((q & 1) ? v.w[0] : (R)1 - v.w[0]) *
+                    (((q >> 1) & 1) ? v.w[1] : (R)1 - v.w[1]) *
+                    ((q >> 2) ? v.w[2] : (R)1 - v.w[2]);
+        atomicAdd(t + (3 * q) * kCells, (double)(W * v.bb[0]));
+        atomicAdd(t + (3 * q + 1) * kCells, (double)(W * v.bb[1]));
+        atomicAdd(t + (3 * q + 2) * kCells, (double)(W * v.bb[2]));
+    }
+}
+
 // Sum the records of one cell bin (keys sorted on the bin bits; the bin's
 // range is found by binary search) into an LDS tile of per-cell corner
 // moments, then add the tile into the global per-cell gradient G[cell][24],
@@ -220,20 +253,7 @@ __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins(const unsigned* _
     const long long r0 = lower(first), r1 = lower(first + (unsigned)kCells);
     for (int k = threadIdx.x; k < kCells * 24; k += blockDim.x) tile[k] = 0.0;
     __syncthreads();
-    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const unsigned local = keys[r] - first;
-        const AdjRec<R> v = vals[r];
-        double* t = tile + local;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const R W = ((q & 1) ? v.w[0] : (R)1 - v.w[0]) *
-                        (((q >> 1) & 1) ? v.w[1] : (R)1 - v.w[1]) *
-                        ((q >> 2) ? v.w[2] : (R)1 - v.w[2]);
-            atomicAdd(t + (3 * q) * kCells, (double)(W * v.bb[0]));
-            atomicAdd(t + (3 * q + 1) * kCells, (double)(W * v.bb[1]));
-            atomicAdd(t + (3 * q + 2) * kCells, (double)(W * v.bb[2]));
-        }
-    }
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) adj_tile_add<R>(tile, keys[r] - first, vals[r]);
     __syncthreads();
     for (int k = threadIdx.x; k < kCells * 24; k += blockDim.x) {
         const unsigned local = (unsigned)(k / 24), comp = (unsigned)(k % 24);

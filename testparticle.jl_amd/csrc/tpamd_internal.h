@@ -40,6 +40,17 @@ hipError_t adj_sort_records(void* d_temp, size_t& temp_bytes, const unsigned* ke
                             unsigned* keys_out, const AdjRec<R>* vals_in, AdjRec<R>* vals_out,
                             long long n, int begin_bit, int end_bit, hipStream_t stream);
 
+// tpamd_sort.hip: the bin sums over block-sorted records
+template <typename R>
+hipError_t adj_bins_seg(const unsigned* keys, const AdjRec<R>* vals, const unsigned* seg_start,
+                        const unsigned* seg_count, int nblk, unsigned nbins, unsigned ncell,
+                        double* G, hipStream_t stream);
+// tpamd_sort.hip: block-local sort of the records by bin (see AdjPart)
+template <typename R>
+hipError_t adj_localsort_records(const unsigned* keys_in, const AdjRec<R>* vals_in, long long M,
+                                 unsigned nbins, unsigned* keys_out, AdjRec<R>* vals_out,
+                                 unsigned* seg_start, unsigned* seg_count, hipStream_t stream);
+
 // records the message on ctx (or as the last tpamd_init error) and returns code
 int fail(tpamd_ctx* ctx, int code, const char* fmt, ...);
 
