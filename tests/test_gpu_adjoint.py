@@ -87,6 +87,28 @@ def test_grid_gradient_vs_central_differences(precision, rtol, nsteps):
     assert worst < rtol
 
 
+def test_grid_gradient_many_bins_radix_path():
+    # 132^3 periodic cells = 4,492 bins of 512 cells: past the 2,048 bins the
+    # block-local partition holds, so the records take the radix sort
+    A, ax, xv0, dt, tspan, tg = _setup(n=132, N=512, nsteps=24)
+    prob, _ = _prob(A, ax, xv0, tspan)
+    r = TP.loss_gradient(prob, dt=dt, targets=tg, precision="fp64", ckpt_interval=8)
+    rng = np.random.default_rng(3)
+    g = np.abs(r.dL_dB)
+    cand = np.argwhere(g > 1e-2 * g.max())
+    picks = cand[rng.choice(len(cand), 8, replace=False)]
+    h = 1e-6 * 5e-9
+    worst = 0.0
+    for c, i, j, k in picks:
+        Ap, Am = A.copy(), A.copy()
+        Ap[c, i, j, k] += h
+        Am[c, i, j, k] -= h
+        fd = (_loss_fp64(Ap, ax, xv0, tspan, dt, tg) - _loss_fp64(Am, ax, xv0, tspan, dt, tg)) / (2 * h)
+        worst = max(worst, abs(r.dL_dB[c, i, j, k] - fd) / abs(fd))
+    print(f"radix path: worst relative error of dL/dB vs central FD on 8 nodes {worst:.2e}")
+    assert worst < 1e-5
+
+
 def test_fp32c_adjoint_rejects_double_storage():
     A, ax, xv0, dt, tspan, tg = _setup(N=8, nsteps=4)
     prob, _ = _prob(A, ax, xv0, tspan, "f64")
