@@ -22,10 +22,8 @@ namespace {
 
 __global__ void k_batch_reset(unsigned* counter, unsigned long long* steps)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *counter = 0u;
-        *steps = 0ull;
-    }
+    if (blockIdx.x == 0 && threadIdx.x < kQueues) counter[threadIdx.x * kQueueStride] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *steps = 0ull;
 }
 
 __global__ void __launch_bounds__(256) k_field_eval(const KParams p, const double* __restrict__ x,
@@ -563,11 +561,6 @@ int batch_time_once(tpamd_batch* b, hipStream_t st, float* ms)
     return TPAMD_OK;
 }
 
-
-// Adjoint driver: forward pass with checkpoints, then per segment (last to
-// first) the reverse sweep, the radix partition of its records by cell bin
-// and the per-bin accumulation; finally the fold of per-cell corner moments
-// into the node gradient.  R is the working precision (float for FP32C).
 
 }  // namespace
 
@@ -1287,14 +1280,16 @@ int32_t tpamd_batch_create(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_table, sizeof(KParams) * n_inst);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_counter, 64);
+    // kQueues claim counters 128 B apart, then the step counter
+    const size_t ctr_bytes = sizeof(unsigned) * kQueues * kQueueStride;
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_counter, ctr_bytes + 128);
     if (e == hipSuccess)
         e = hipMemcpy(b->d_table, b->table.data(), sizeof(KParams) * n_inst, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         tpamd_batch_free(b);
         return fail(ctx, TPAMD_ERR_DEVICE, "batch set-up failed: %s", hipGetErrorString(e));
     }
-    b->d_steps = (unsigned long long*)((char*)b->d_counter + 8);
+    b->d_steps = (unsigned long long*)((char*)b->d_counter + ctr_bytes);
     b->n_cu = prop.multiProcessorCount;
     int rc = batch_chunks(b, b->ppt, 1);
     if (!rc) rc = batch_capture(b);
@@ -1339,7 +1334,9 @@ int32_t tpamd_batch_autotune(tpamd_batch* b, void* stream, int32_t* block_size, 
     tpamd_ctx* ctx = b->ctx;
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    // time each candidate on every 8th chunk of the full chunk list
+    // time each candidate on the full chunk list: a sample of the chunks
+    // under-fills the GPU at large ppt (fewer chunks than resident waves) and
+    // so biases the choice toward small ppt
     static const int kBlocks[] = {64, 128, 256};
     static const int kPpt[] = {1, 2, 4, 8, 16, 32};
     float best = 3.0e38f;
@@ -1347,7 +1344,7 @@ int32_t tpamd_batch_autotune(tpamd_batch* b, void* stream, int32_t* block_size, 
     for (int bs : kBlocks) {
         for (int pp : kPpt) {
             b->block_size = bs;
-            int rc = batch_chunks(b, pp, 8);
+            int rc = batch_chunks(b, pp, 1);
             float ms = 0.f;
             if (!rc) rc = batch_time_once(b, st, &ms);
             if (rc) return rc;

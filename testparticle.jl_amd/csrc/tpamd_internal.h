@@ -61,6 +61,15 @@ int fail(tpamd_ctx* ctx, int code, const char* fmt, ...);
 // stride over its particles (particles per thread = chunk size / 64), which
 // evens out uneven lifetimes.  Every wave leaves once the queue is drained.
 // ---------------------------------------------------------------------------
+// The batched kernel's chunk list is dealt into kQueues interleaved slices
+// (slice q: chunks q, q + kQueues, ...), each with its own claim counter on
+// its own 128-B line.  A wave starts on the slice of its block's XCD
+// (blocks are dispatched to the 8 XCDs round-robin) and moves on to the
+// others when it is drained, so all waves finish; one shared counter
+// serialised ~8 claims per microsecond.
+constexpr int kQueues = 8;
+constexpr int kQueueStride = 32;   // unsigned words between counters (128 B)
+
 struct Chunk {
     int inst;
     int pad;

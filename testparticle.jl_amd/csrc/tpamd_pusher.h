@@ -347,49 +347,66 @@ __device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
     return final_step;
 }
 
-// _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266
+// _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266, as a
+// resumable particle: init (the prologue, :160-195), step (one pass of the
+// loop, :197-248; false once the particle is done) and finish (the closing
+// record, :250-259).  run_adaptive runs it to the end; the batched kernel
+// interleaves many, handing a lane a new particle as soon as its previous one
+// is done.  Either way a particle sees the same operations in the same order.
 template <class S, int ALG, bool REC>
-__device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
-{
+struct AdaptiveRun {
     using R = typename S::R;
     S s;
-    s.load(p, i);
-    const double t0 = p.t0, t1 = p.t1;
-    const double C = p.C;                                     // :164
-    const double ttotal = t1 - t0;
-    double t = t0;
-    long long nrec = 0;
-    double tlast = 0.0;
-    R vt[3];
-    auto push = [&](const R vv[3], double tt) {
+    long long i, it, next_save, nrec, cap_steps;
+    double t, dt, tlast;
+    bool should;
+    int ret;
+    // the fields at the current (x, t).  The reference evaluates there up to
+    // four times per step (|B| for dt, both halves of the resynchronisation,
+    // a save); one evaluation serves them all, and with static fields the
+    // mid-step update as well.
+    R Ec[3], Bc[3];
+
+    __device__ __forceinline__ void push(const KParams& p, const R vv[3], double tt)
+    {
         if (nrec < p.cap) record<REC>(p, s, i, nrec, vv, tt);
         tlast = tt;
         nrec += 1;
-    };
-    if (p.save_start) push(s.v, t);                           // :179-186
-    // Ec, Bc: the fields at the current (x, t).  The reference evaluates
-    // there up to four times per step (|B| for dt, both halves of the
-    // resynchronisation, a save); one evaluation serves them all, and with
-    // static fields the mid-step update as well.
-    R Ec[3], Bc[3];
-    s.eval(p, t, Ec, Bc);
-    double dt = C / s.bmag_of(Bc);                            // :188-189
-    s.set_dt(dt);
-    s.template Uf<ALG>(p, Ec, Bc, s.v, -0.5 * dt, vt);        // :192
+    }
+
+    __device__ __forceinline__ void init(const KParams& p, long long idx)
+    {
+        i = idx;
+        s.load(p, i);
+        t = p.t0;
+        nrec = 0;
+        tlast = 0.0;
+        if (p.save_start) push(p, s.v, t);                   // :179-186
+        s.eval(p, t, Ec, Bc);
+        dt = p.C / s.bmag_of(Bc);                             // :164, :188-189
+        s.set_dt(dt);
+        R vt[3];
+        s.template Uf<ALG>(p, Ec, Bc, s.v, -0.5 * dt, vt);    // :192
 #pragma unroll
-    for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
-    long long it = 1;
-    long long next_save = p.save_every ? p.K + 1 : -1;
-    bool should = p.save_end;
-    int ret = kRetSuccess;
-    // every wave must leave the loop: a zero step (|B| = inf) or an
-    // unbounded run stops with MaxIters instead of spinning
-    const long long cap_steps = p.max_steps > 0 ? p.max_steps : kAdaptiveHardCap;
-    while (fabs(t - t0) < fabs(ttotal)) {                     // :197
+        for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+        it = 1;
+        next_save = p.save_every ? p.K + 1 : -1;
+        should = p.save_end;
+        ret = kRetSuccess;
+        // every particle must stop: a zero step (|B| = inf) or an unbounded
+        // run ends with MaxIters instead of spinning
+        cap_steps = p.max_steps > 0 ? p.max_steps : kAdaptiveHardCap;
+    }
+
+    __device__ __forceinline__ bool step(const KParams& p)
+    {
+        const double t0 = p.t0, t1 = p.t1, ttotal = t1 - t0;
+        if (!(fabs(t - t0) < fabs(ttotal))) return false;    // :197
         if (dt == 0.0) {
             ret = kRetMaxIters;
-            break;
+            return false;
         }
+        R vt[3];
         if (fabs(t + dt - t0) > fabs(ttotal)) {               // :198-203
             const double dts = t1 - t;
             s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
@@ -400,7 +417,7 @@ __device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
         if (it == next_save) {                                // :205-218
             next_save += p.K;
             s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
-            push(vt, t);
+            push(p, vt, t);
         }
         const double tmid = t + 0.5 * dt;                     // :220
         R vn[3];
@@ -412,14 +429,14 @@ __device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
         if (outside(p, s, xn, vn)) {                          // :227-231
             should = true;
             ret = kRetTerminated;
-            break;
+            return false;
         }
         s.pos = xn;                                           // :233-235
         t = tn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.v[c] = vn[c];
         s.eval(p, t, Ec, Bc);
-        const double dtn = C / s.bmag_of(Bc);                 // :237-238
+        const double dtn = p.C / s.bmag_of(Bc);               // :237-238
         s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);     // :241-242
         s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dtn, s.v);
         dt = dtn;
@@ -428,16 +445,32 @@ __device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
         if (it == next_save) should = true;                   // :245-247, (it-1) % K == 0
         if (it - 1 >= cap_steps) {
             ret = kRetMaxIters;
-            break;
+            return false;
         }
+        return true;
     }
-    if (should && (nrec == 0 || tlast != t)) {                // :250-259
-        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
-        push(vt, t);
+
+    __device__ __forceinline__ long long finish(const KParams& p)
+    {
+        if (should && (nrec == 0 || tlast != t)) {            // :250-259
+            R vt[3];
+            s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
+            push(p, vt, t);
+        }
+        p.out_len[i] = (int)nrec;
+        p.retcode[i] = ret;
+        return it - 1;
     }
-    p.out_len[i] = (int)nrec;
-    p.retcode[i] = ret;
-    return it - 1;
+};
+
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
+{
+    AdaptiveRun<S, ALG, REC> r;
+    r.init(p, i);
+    while (r.step(p)) {
+    }
+    return r.finish(p);
 }
 
 __device__ __forceinline__ void add_steps(const KParams& p, long long steps)
@@ -561,6 +594,44 @@ struct S32G : S32C {
     }
 };
 
+// The particles [begin, end) of one chunk (one field instance), adaptive:
+// lives are uneven (a particle may leave after one step or run to the step
+// cap), so instead of giving each lane a fixed list, which holds the wave
+// until the longest life of every round ends, a lane takes the chunk's next
+// particle as soon as its own is done.  The hand-out is a wave-uniform
+// cursor advanced by a ballot: no atomics.
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long adaptive_chunk(const KParams& p, long long begin, long long end,
+                                                    int lane)
+{
+    long long steps = 0;
+    AdaptiveRun<S, ALG, REC> r;
+    bool active = begin + lane < end;
+    if (active) r.init(p, begin + lane);
+    long long next = begin + 64;                          // wave-uniform cursor
+    const unsigned long long below = (1ull << lane) - 1ull;
+    while (true) {
+        if (active && !r.step(p)) {
+            steps += r.finish(p);
+            active = false;
+        }
+        const unsigned long long idle = __ballot(!active);
+        if (next >= end) {
+            if (idle == ~0ull) break;                     // nothing left to hand out, all done
+            continue;
+        }
+        if (!active) {
+            const long long idx = next + (long long)__popcll(idle & below);
+            if (idx < end) {
+                r.init(p, idx);
+                active = true;
+            }
+        }
+        next += (long long)__popcll(idle);
+    }
+    return steps;
+}
+
 template <class S, int ALG, bool ADAPT, bool REC = false>
 __global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table,
                                                const Chunk* __restrict__ chunks, int nchunks,
@@ -572,20 +643,27 @@ __global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table
     // the chunk index is made wave-uniform (readfirstlane), so the chunk and
     // the instance's parameter block are read with scalar loads; the next
     // chunk is claimed before the current one is worked on, which hides the
-    // atomic's round trip
-    unsigned claim = 0;
-    if (lane == 0) claim = atomicAdd(counter, 1u);
-    unsigned c = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
-    while (c < (unsigned)nchunks) {
-        unsigned next = 0;
-        if (lane == 0) next = atomicAdd(counter, 1u);
-        const Chunk ch = chunks[c];
-        const KParams& p = table[ch.inst];
-        for (long long i = ch.begin + lane; i < ch.end; i += 64) {
-            if constexpr (ADAPT) steps += run_adaptive<S, ALG, REC>(p, i);
-            else steps += run_fixed<S, ALG, REC>(p, i);
+    // atomic's round trip.  Slices: see kQueues.
+    const unsigned q0 = blockIdx.x % kQueues;
+    for (int j = 0; j < kQueues; ++j) {
+        const unsigned q = (q0 + (unsigned)j) % kQueues;
+        const unsigned nq = (unsigned)nchunks > q ? ((unsigned)nchunks - q + kQueues - 1) / kQueues : 0u;
+        unsigned* ctr = counter + q * kQueueStride;
+        unsigned claim = 0;
+        if (lane == 0) claim = atomicAdd(ctr, 1u);
+        unsigned c = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
+        while (c < nq) {
+            unsigned next = 0;
+            if (lane == 0) next = atomicAdd(ctr, 1u);
+            const Chunk ch = chunks[q + c * kQueues];
+            const KParams& p = table[ch.inst];
+            if constexpr (ADAPT) {
+                steps += adaptive_chunk<S, ALG, REC>(p, ch.begin, ch.end, lane);
+            } else {
+                for (long long i = ch.begin + lane; i < ch.end; i += 64) steps += run_fixed<S, ALG, REC>(p, i);
+            }
+            c = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
         }
-        c = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
     }
     unsigned long long v = (unsigned long long)steps;
 #pragma unroll
