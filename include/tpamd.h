@@ -131,7 +131,10 @@ typedef struct tpamd_solve_desc {
     double mass;              /* p[2] = m, for save_work (src/boris/boris.jl:405-440) */
     int32_t save_fields;      /* records gain E(x, t) and B(x, t): 6 + 6 columns */
     int32_t save_work;        /* records gain (P_par, P_fermi, P_grad, P_betatron) */
-    int32_t reserved[2];
+    int32_t particles_per_lane; /* adaptive solves only: each wave works through 64 x this many
+                                   particles, a lane taking the next one when its own ends;
+                                   0 = library choice (a batch has its own, tpamd_batch_autotune) */
+    int32_t reserved;
 } tpamd_solve_desc;
 
 /* Columns of one record: 6 (x, v) + 6 when save_fields + 4 when save_work,

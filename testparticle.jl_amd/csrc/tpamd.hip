@@ -394,9 +394,21 @@ int solve_impl(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* 
     const int bs = desc->block_size > 0 ? desc->block_size : 256;
     if (bs % 64 != 0 || bs > 256)
         return fail(ctx, TPAMD_ERR_ARGUMENT, "block_size must be a multiple of 64 up to 256");
+    // adaptive: a wave takes p.span particles.  More particles per lane let
+    // lanes refill when lives are uneven but leave fewer waves; by default a
+    // lane takes one particle until the grid holds 4,096 waves (16 per CU),
+    // and at most 8
+    const int ppl_req = desc->particles_per_lane;
+    if (ppl_req < 0 || ppl_req > 1024)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "particles_per_lane must be in [0, 1024]");
+    if (ppl_req > 0 && !desc->adaptive)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "particles_per_lane applies to adaptive solves only");
+    const long long ppl = ppl_req > 0 ? ppl_req : std::max(1LL, std::min(8LL, (long long)N / (64LL * 4096)));
+    p.span = (int)(64 * ppl);
     float ms = 0.f;
     if (N > 0) {
-        const dim3 block(bs), grid((unsigned)((N + bs - 1) / bs));
+        const long long threads = desc->adaptive ? (N + p.span - 1) / p.span * 64 : N;
+        const dim3 block(bs), grid((unsigned)((threads + bs - 1) / bs));
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
         LaunchReq r{};
         r.batch = false;

@@ -94,7 +94,7 @@ struct KParams {
     int static_fields;        // neither E nor B depends on t
     int rec_w;                // record columns: 6, + 6 (E, B), + 4 (work rates)
     int save_fields, save_work;
-    int pad1;
+    int span;                 // adaptive solves: particles per wave (64 x particles per lane)
     double mass;              // for the work rates
     double q2m, t0, t1, dt, C;
     long long nt, nout, cap, K, max_steps, N;

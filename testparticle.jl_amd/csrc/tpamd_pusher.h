@@ -491,12 +491,54 @@ __global__ void __launch_bounds__(256) k_fixed(const KParams p)
     add_steps(p, steps);
 }
 
+// Adaptive particles [begin, end) of one problem on one wave: lives are
+// uneven (a particle may leave after one step or run to the step cap), so
+// instead of giving each lane a fixed list, which holds the wave until the
+// longest life of every round ends, a lane takes the range's next particle
+// as soon as its own is done.  The hand-out is a wave-uniform
+// cursor advanced by a ballot: no atomics.
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long adaptive_chunk(const KParams& p, long long begin, long long end,
+                                                    int lane)
+{
+    long long steps = 0;
+    AdaptiveRun<S, ALG, REC> r;
+    bool active = begin + lane < end;
+    if (active) r.init(p, begin + lane);
+    long long next = begin + 64;                          // wave-uniform cursor
+    const unsigned long long below = (1ull << lane) - 1ull;
+    while (true) {
+        if (active && !r.step(p)) {
+            steps += r.finish(p);
+            active = false;
+        }
+        const unsigned long long idle = __ballot(!active);
+        if (next >= end) {
+            if (idle == ~0ull) break;                     // nothing left to hand out, all done
+            continue;
+        }
+        if (!active) {
+            const long long idx = next + (long long)__popcll(idle & below);
+            if (idx < end) {
+                r.init(p, idx);
+                active = true;
+            }
+        }
+        next += (long long)__popcll(idle);
+    }
+    return steps;
+}
+
+// one wave per p.span consecutive particles, refilled lane by lane
 template <class S, int ALG, bool REC = false>
 __global__ void __launch_bounds__(256) k_adaptive(const KParams p)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long begin = w * (long long)p.span;
     long long steps = 0;
-    if (i < p.N) steps = run_adaptive<S, ALG, REC>(p, i);
+    if (begin < p.N)
+        steps = adaptive_chunk<S, ALG, REC>(p, begin, begin + p.span < p.N ? begin + p.span : p.N,
+                                            (int)(threadIdx.x & 63));
     add_steps(p, steps);
 }
 
@@ -593,44 +635,6 @@ struct S32G : S32C {
         Uf<ALG>(p, E, B, vin, h, out);
     }
 };
-
-// The particles [begin, end) of one chunk (one field instance), adaptive:
-// lives are uneven (a particle may leave after one step or run to the step
-// cap), so instead of giving each lane a fixed list, which holds the wave
-// until the longest life of every round ends, a lane takes the chunk's next
-// particle as soon as its own is done.  The hand-out is a wave-uniform
-// cursor advanced by a ballot: no atomics.
-template <class S, int ALG, bool REC>
-__device__ __forceinline__ long long adaptive_chunk(const KParams& p, long long begin, long long end,
-                                                    int lane)
-{
-    long long steps = 0;
-    AdaptiveRun<S, ALG, REC> r;
-    bool active = begin + lane < end;
-    if (active) r.init(p, begin + lane);
-    long long next = begin + 64;                          // wave-uniform cursor
-    const unsigned long long below = (1ull << lane) - 1ull;
-    while (true) {
-        if (active && !r.step(p)) {
-            steps += r.finish(p);
-            active = false;
-        }
-        const unsigned long long idle = __ballot(!active);
-        if (next >= end) {
-            if (idle == ~0ull) break;                     // nothing left to hand out, all done
-            continue;
-        }
-        if (!active) {
-            const long long idx = next + (long long)__popcll(idle & below);
-            if (idx < end) {
-                r.init(p, idx);
-                active = true;
-            }
-        }
-        next += (long long)__popcll(idle);
-    }
-    return steps;
-}
 
 template <class S, int ALG, bool ADAPT, bool REC = false>
 __global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table,

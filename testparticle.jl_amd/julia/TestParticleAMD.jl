@@ -58,7 +58,8 @@ Base.@kwdef mutable struct SolveDesc
     mass::Float64 = 0.0
     save_fields::Int32 = 0
     save_work::Int32 = 0
-    reserved::NTuple{2, Int32} = ntuple(_ -> Int32(0), 2)
+    particles_per_lane::Int32 = 0
+    reserved::Int32 = 0
 end
 
 mutable struct Stats

@@ -58,7 +58,7 @@ class SolveDesc(C.Structure):
         ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
         ("dt_pp", C.c_void_p), ("nsteps_pp", C.c_int64), ("block_size", C.c_int32),
         ("flags", C.c_int32), ("mass", C.c_double), ("save_fields", C.c_int32),
-        ("save_work", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("save_work", C.c_int32), ("particles_per_lane", C.c_int32), ("reserved", C.c_int32),
     ]
 
     @property

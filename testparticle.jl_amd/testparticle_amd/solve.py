@@ -67,7 +67,8 @@ def make_desc(prob: TraceProblem, alg: AbstractBoris, *, dt=None, savestepinterv
               isoutside=None, save_start=True, save_end=True, save_everystep=True,
               maxiters=1_000_000, precision="fp64", form="rational", max_steps=0,
               nout_cap=0, dts=None, nsteps=0, workgroup_size=0,
-              kernels="auto", save_fields=False, save_work=False) -> _lib.SolveDesc:
+              kernels="auto", save_fields=False, save_work=False,
+              particles_per_lane=0) -> _lib.SolveDesc:
     q2m = float(prob.p[0])
     d = _lib.SolveDesc()
     # records gain E, B and/or the work rates (src/boris/boris.jl:405-440);
@@ -103,6 +104,7 @@ def make_desc(prob: TraceProblem, alg: AbstractBoris, *, dt=None, savestepinterv
     d.max_steps = int(max_steps)
     d.nout_cap = int(nout_cap)
     d.block_size = int(workgroup_size or 0)
+    d.particles_per_lane = int(particles_per_lane or 0)
     if kernels not in ("auto", "generic"):
         raise _lib.ArgumentError("kernels must be 'auto' or 'generic'")
     d.flags = _lib.FLAG_GENERIC_KERNELS if kernels == "generic" else 0
@@ -156,14 +158,17 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
           save_work: bool = False, maxiters: int = 1_000_000, batch_size=None, seed=None,
           workgroup_size: int = 0, precision: str = "fp64", form: str = "rational",
           max_steps: int = 0, nout_cap=None, dts=None, nsteps: int = 0,
-          kernels: str = "auto", device=None) -> EnsembleSolution:
+          kernels: str = "auto", particles_per_lane: int = 0, device=None) -> EnsembleSolution:
     """Trace an ensemble with the Boris family on the GPU.
 
     Extensions beyond the reference's keywords: `precision` ("fp64" |
     "fp32c"), `form` ("rational" | "reference", multistep coefficients),
     `max_steps` (adaptive step cap), `dts` + `nsteps` (per-particle step,
     fixed-step only), `nout_cap` (adaptive record capacity), `kernels`
-    ("auto" | "generic": skip the gridded-field specialisations)."""
+    ("auto" | "generic": skip the gridded-field specialisations),
+    `particles_per_lane` (adaptive only: each wave of 64 lanes works through
+    64 x this many particles, a lane taking the next when its own ends; 0 =
+    the library's choice from N)."""
     backend, _ens = _split_positional(args)
     if alg.adaptive and dt is not None:
         raise TypeError("the adaptive solver takes no dt (src/boris/adaptive_boris.jl:21-33)")
@@ -178,7 +183,8 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
                      save_start=save_start, save_end=save_end, save_everystep=save_everystep,
                      maxiters=maxiters, precision=precision, form=form, max_steps=max_steps,
                      dts=dts_arr, nsteps=nsteps, workgroup_size=workgroup_size,
-                     kernels=kernels, save_fields=save_fields, save_work=save_work)
+                     kernels=kernels, save_fields=save_fields, save_work=save_work,
+                     particles_per_lane=particles_per_lane)
     if dts_arr is not None:
         desc.dt_pp = dts_arr.ctypes.data
     if not alg.adaptive:

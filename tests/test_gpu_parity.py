@@ -402,6 +402,43 @@ def test_parity_adaptive_open_box(ms):
     assert_bitwise(g, o)
 
 
+@pytest.mark.parametrize("ppl", [1, 3, 8])
+def test_adaptive_lane_refill_bitwise(ppl):
+    # a wave works through 64 x ppl particles, a lane taking the next one as
+    # soon as its own leaves the box or reaches the step cap; N = 1000 leaves
+    # a ragged last wave at every ppl.  Results cannot depend on the hand-out.
+    w, A = W.turbulent_box(N=1000, n=32, nsteps=1, dtype=np.float64, seed=12)
+    L = w.extra["L"]
+    ax = W.grid_axes(32, L, periodic=False)
+    B = TP.GridField(A, *ax, bc=TP.FillExtrap())
+    prob = TP.TraceProblem(w.xv0, (0.0, 400 * w.dt), (w.q2m, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+    alg = TP.AdaptiveBoris(safety=0.02)
+    kw = dict(isoutside=TP.OutsideBox((0, 0, 0), (L, L, L)), savestepinterval=5, max_steps=40,
+              nout_cap=16)
+    g = TP.solve(prob, alg, particles_per_lane=ppl, **kw)
+    # exits (most after a few steps) and step caps both occur: lives are uneven
+    assert (g.retcodes == 1).any() and (g.retcodes == 2).any()
+    assert_bitwise(g, run_oracle(prob, alg, **kw))
+    f = TP.solve(prob, alg, precision="fp32c", particles_per_lane=ppl, **kw)
+    f1 = TP.solve(prob, alg, precision="fp32c", particles_per_lane=1, **kw)
+    np.testing.assert_array_equal(f.lengths, f1.lengths)
+    np.testing.assert_array_equal(f.u_all, f1.u_all)
+    np.testing.assert_array_equal(f.t_all, f1.t_all)
+
+
+def test_particles_per_lane_checks():
+    z = TP.ZeroField()
+    prob = TP.TraceProblem(np.zeros((4, 6)) + [0, 0, 0, 1, 0, 0], (0.0, 1.0),
+                           (1.0, 1.0, z, TP.UniformField([0, 0, 1.0]), z))
+    with pytest.raises(TP.ArgumentError, match="adaptive solves only"):
+        TP.solve(prob, TP.Boris(), dt=0.1, particles_per_lane=2)
+    with pytest.raises(TP.ArgumentError, match="particles_per_lane"):
+        TP.solve(prob, TP.AdaptiveBoris(), particles_per_lane=-1)
+    # more lanes than particles: idle lanes, same answer
+    a = TP.solve(prob, TP.AdaptiveBoris(), particles_per_lane=16)
+    np.testing.assert_array_equal(a.u_all, TP.solve(prob, TP.AdaptiveBoris()).u_all)
+
+
 def test_parity_affine_time_dependent_all_orders():
     rng = np.random.default_rng(5)
     N = 256

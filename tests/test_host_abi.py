@@ -192,6 +192,8 @@ def test_record_width_and_mass_in_descriptor():
         assert d.mass == 7.0 and d.q2m == 2.5
     # the C struct keeps its size: the new fields took the reserved words
     assert C.sizeof(TP._lib.SolveDesc) == 224
+    assert TP._lib.SolveDesc.particles_per_lane.offset == 216
+    assert TP.make_desc(prob, TP.AdaptiveBoris(), particles_per_lane=4).particles_per_lane == 4
 
 
 def test_solve_batch_argument_checks():
