@@ -838,6 +838,40 @@ __device__ __forceinline__ void pos_abs(const KParams& p, const Pos32& x, double
                ((double)x.hi[d] + (double)x.lo[d]);
 }
 
+// The same per field-kind class.  Solves of the analytic class have no
+// frame (origin 0, no wraps: only grids set one), so the absolute position
+// is the double sum hi + lo; the leading +0.0 keeps the general form's sign
+// of an exact zero.
+template <int FK>
+__device__ __forceinline__ void pos_abs_t(const KParams& p, const Pos32& x, double o[3])
+{
+    if constexpr (FK == kFkAnalytic) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) o[d] = 0.0 + ((double)x.hi[d] + (double)x.lo[d]);
+    } else {
+        pos_abs(p, x, o);
+    }
+}
+
+// the absolute position rounded to float, for fields evaluated in float.
+// Without a frame it is the float sum hi + lo: rounding a sum of two floats
+// first to double and then to float gives the correctly rounded float sum
+// (double rounding is harmless for + when 53 >= 2 x 24 + 1; S. A. Figueroa,
+// "When is double rounding innocuous?", SIGNUM Newsletter 30(3), 1995).
+template <int FK>
+__device__ __forceinline__ void pos_f32(const KParams& p, const Pos32& x, float o[3])
+{
+    if constexpr (FK == kFkAnalytic) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) o[d] = 0.0f + (x.hi[d] + x.lo[d]);
+    } else {
+        double xa[3];
+        pos_abs(p, x, xa);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) o[d] = (float)xa[d];
+    }
+}
+
 template <int FK = kFkMesh>
 __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, const Pos32& x,
                                           double t, float o[3])
@@ -890,9 +924,8 @@ __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, c
         return;
     }
     case kFieldAffine: {
-        double xa[3];
-        pos_abs(p, x, xa);
-        float xf[3] = {(float)xa[0], (float)xa[1], (float)xa[2]};
+        float xf[3];
+        pos_f32<FK>(p, x, xf);
         const float tf = (float)t;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -904,9 +937,8 @@ __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, c
         return;
     }
     case kFieldDipole: {
-        double xa[3];
-        pos_abs(p, x, xa);
-        const float xf[3] = {(float)xa[0], (float)xa[1], (float)xa[2]};
+        float xf[3];
+        pos_f32<FK>(p, x, xf);
         // b0 (R/r)^3 / r^2 keeps every intermediate inside float's normal range
         const float r2 = dot3f(xf, xf);
         const float ir = rsqrtf(r2);

@@ -173,7 +173,7 @@ struct S32CT {
             xn.wrap[c] = w;
         }
     }
-    __device__ __forceinline__ void abs_pos(const KParams& p, const Pos& q, double o[3]) const { pos_abs(p, q, o); }
+    __device__ __forceinline__ void abs_pos(const KParams& p, const Pos& q, double o[3]) const { pos_abs_t<FK>(p, q, o); }
     __device__ __forceinline__ void vel_d(const float vv[3], double o[3]) const
     {
         o[0] = (double)vv[0]; o[1] = (double)vv[1]; o[2] = (double)vv[2];
