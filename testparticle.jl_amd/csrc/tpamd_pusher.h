@@ -30,6 +30,7 @@ template <int FK>
 struct S64T {
     static constexpr int kNS = -1;   // generic: any algorithm, adaptive or not
     static constexpr int kFK = FK;   // field-kind class (kFk*)
+    static constexpr int kWaves = 1; // waves per SIMD the fixed-step kernel must fit (1: no bound)
     using R = double;
     struct Pos { double x[3]; };
     Pos pos;
@@ -90,6 +91,7 @@ template <int FK>
 struct S32CT {
     static constexpr int kNS = -1;
     static constexpr int kFK = FK;
+    static constexpr int kWaves = 1;
     using R = float;
     using Pos = Pos32;
     Pos pos;
@@ -483,7 +485,7 @@ __device__ __forceinline__ void add_steps(const KParams& p, long long steps)
 }
 
 template <class S, int ALG, bool REC = false>
-__global__ void __launch_bounds__(256) k_fixed(const KParams p)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S::kWaves))) k_fixed(const KParams p)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long steps = 0;
@@ -549,6 +551,12 @@ __global__ void __launch_bounds__(256) k_adaptive(const KParams p)
 template <int EMODE, int NS, bool PER, typename T, bool CELLS>
 struct S64G : S64 {
     static constexpr int kNS = NS;
+    // FP64 periodic cells with E = -u x B: bounding the kernel to 96 VGPRs
+    // (5 waves per SIMD instead of 4) hides more of the gather latency and
+    // beats the 44-56 B it spills (+10 % Boris, +4 % multistep at 256^3).
+    // With a stored uniform E the same bound spills 100-136 B and loses 20 %,
+    // and the non-periodic variants would spill 76-84 B: no bound there.
+    static constexpr int kWaves = (EMODE == kEUxB && PER && CELLS && sizeof(T) == 8) ? 5 : 1;
     __device__ __forceinline__ void bfield(const KParams& p, double B[3]) const
     {
         int i0[3], i1[3];
