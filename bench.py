@@ -44,6 +44,9 @@ FP32_PEAK = 157.3e12       # FLOP/s, vector FP32 (packed)
 # algorithmic bytes per particle-step of the gather: one vector field,
 # 8 nodes x 3 components (E = -u x B derived in-kernel)
 GATHER_BYTES = {"f32": 8 * 3 * 4, "f64": 8 * 3 * 8}
+# per particle of a solve: its initial state (6 doubles) read, its length and
+# return code (2 x int32) written; per record: x, v (6 doubles) and t
+STATE_BYTES, LEN_BYTES, RECORD_BYTES = 6 * 8, 2 * 4, 7 * 8
 # algorithmic FLOPs per particle-step (FMA = 2; div, sqrt = 1): standard Boris
 # with E = 0 (37) + position update (6) + dipole evaluation (27)
 DIPOLE_FLOP = 37 + 6 + 27
@@ -302,13 +305,22 @@ def bench_batch(args, torch, TP, W):
         wall, kms, st = timed_runs(b.run, torch, 1, args.aux_steps)
         fin[prec] = b.final_states()[:, :3]
         rate = st.particle_steps / wall
-        achieved = st.particle_steps * GATHER_BYTES["f32" if prec == "fp32c" else "f64"] / (kms * 1e-3)
+        # particles live ~1.4 steps, so the per-particle state and record
+        # bytes weigh as much as the per-step gathers (float grids: 96 B)
+        nrec = int(b.len.sum().item())
+        algo = (st.particle_steps * GATHER_BYTES["f32"] + N * (STATE_BYTES + LEN_BYTES)
+                + nrec * RECORD_BYTES)
+        achieved = algo / (kms * 1e-3)
         out[prec] = {"value": rate, "unit": "particle-steps/s", "ms_per_run": wall * 1e3,
                      "kernel_ms": kms, "particle_steps": int(st.particle_steps),
+                     "particles_per_s": N / wall, "records": nrec,
                      "autotuned": {"block": bs, "particles_per_lane": ppt},
                      "roofline": {"bound": "hbm", "achieved": achieved / 1e9,
                                   "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                                  "frac": achieved / HBM_PEAK, "kernel": "k_batch (adaptive)"}}
+                                  "frac": achieved / HBM_PEAK, "kernel": "k_batch (adaptive)",
+                                  "algorithmic_bytes_per_launch": int(algo),
+                                  "bytes": "96 B per step (float cell gather) + 56 B per particle "
+                                           "(state in, length and return code) + 56 B per record"}}
         log(f"[bench] batch {prec}: {rate:.3e} particle-steps/s, block {bs}, ppt {ppt}")
         b.close()
         del b
