@@ -83,12 +83,40 @@ __global__ void __launch_bounds__(AdjPart::kThreads) k_adj_localsort(
     }
 }
 
-// As k_adj_bins, over block-sorted records: bin b's records are the
+// Index of the f-th record of a bin's concatenated segments within a chunk
+// of blocks starting at block c0: pre[] is the exclusive scan of the
+// segment counts and beg[] the segments' block-local starts (nb entries).
+template <int P>
+__device__ __forceinline__ long long seg_record(const unsigned* pre, const unsigned* beg, int nb, int c0,
+                                                unsigned f)
+{
+    int lo = 0, hi = nb - 1;                  // the last segment starting at or before f
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= f) lo = mid;
+        else hi = mid - 1;
+    }
+    return (long long)(c0 + lo) * P + beg[lo] + (f - pre[lo]);
+}
+
+// The sums of k_adj_bins, over block-sorted records: bin b's records are the
 // segments [blk * P + seg_start[b][blk], + seg_count[b][blk]) of every block.
 // The bin's row of segment counts is scanned in LDS, kChunk blocks at a time,
-// and thread f takes the f-th record of the concatenated segments (found by
-// binary search in the scan), so neighbouring threads read neighbouring
-// records of a segment.
+// and thread f takes the f-th record of the concatenated segments, so
+// neighbouring threads read neighbouring records of a segment.
+//
+// Double records are summed into the 24 corner moments of their cell as LDS
+// atomics (ds_add_f64) on a tile of the bin's cells.  For float records that
+// capped the kernel at the LDS atomic rate, so there each batch of 4,096
+// records is grouped by cell in LDS with a counting sort (one ds_add_u32 per
+// record for its rank in the cell, a block scan of the cell counts), and two
+// owner threads per cell, one for corners 0-3 and one for 4-7, sum the
+// cell's records into double registers: 263 us against 347 us a call at the
+// 64^3 adjoint workload.  Staging double records the same way (2,048 a batch,
+// 64-bit LDS accesses) lost: 412 us against 354 us.  Either way each product
+// is formed in the record's precision and converted to double; only the
+// order of the double sums differs.  At the end the workgroup adds its sums
+// to the per-cell gradient G[cell][24], which it alone writes.
 template <typename R>
 __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins_seg(
     const unsigned* __restrict__ keys, const AdjRec<R>* __restrict__ vals,
@@ -100,16 +128,34 @@ __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins_seg(
     constexpr int P = AdjPart::part<R>();
     constexpr int kChunk = 4096;
     constexpr int IPT = kChunk / T;
+    constexpr bool OWN = sizeof(R) == 4;      // owner sums (float) or LDS atomics (double)
+    constexpr int RB = 4096;                  // owner sums: records grouped per batch
+    constexpr int RPT = RB / T;
+    static_assert(T == 2 * kCells, "two owner threads per cell");
+    static_assert(6 * RB * sizeof(float) == kCells * 24 * sizeof(double), "one LDS buffer for both");
     using Scan = hipcub::BlockScan<unsigned, T>;
     __shared__ typename Scan::TempStorage scan_tmp;
-    __shared__ double tile[kCells * 24];
     __shared__ unsigned pre[kChunk];
     __shared__ unsigned beg[kChunk];
+    __shared__ unsigned cnt[OWN ? kCells : 1];
+    __shared__ unsigned off[OWN ? kCells : 1];
+    // the atomics' tile[(3 q + c) * kCells + cell], or the owners' staged
+    // records sb[v * RB + slot] (v: Bbar 0-2, weights 3-5)
+    __shared__ double lds[kCells * 24];
+    double* tile = lds;
+    float* sb = reinterpret_cast<float*>(lds);
     const int tid = threadIdx.x;
-    const unsigned bin = blockIdx.x;
-    const unsigned first = bin << AdjBin::kShift;
-    for (int k = tid; k < kCells * 24; k += T) tile[k] = 0.0;
-    const size_t row = (size_t)bin * (size_t)nblk;
+    const unsigned first = blockIdx.x << AdjBin::kShift;
+    const int own = tid >> 1, half = tid & 1;
+    double acc[OWN ? 12 : 1];
+    if constexpr (OWN) {
+#pragma unroll
+        for (int j = 0; j < 12; ++j) acc[j] = 0.0;
+        if (tid < kCells) cnt[tid] = 0u;
+    } else {
+        for (int k = tid; k < kCells * 24; k += T) tile[k] = 0.0;
+    }
+    const size_t row = (size_t)blockIdx.x * (size_t)nblk;
     for (int c0 = 0; c0 < nblk; c0 += kChunk) {
         const int nb = nblk - c0 < kChunk ? nblk - c0 : kChunk;
         unsigned v[IPT];
@@ -128,22 +174,80 @@ __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins_seg(
             beg[i] = i < nb ? seg_start[row + c0 + i] : 0u;
         }
         __syncthreads();
-        for (unsigned f = tid; f < total; f += T) {
-            int lo = 0, hi = nb - 1;          // the last segment starting at or before f
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (pre[mid] <= f) lo = mid;
-                else hi = mid - 1;
+        if constexpr (!OWN) {
+            for (unsigned f = tid; f < total; f += T) {
+                const long long r = seg_record<P>(pre, beg, nb, c0, f);
+                adj_tile_add<R>(tile, keys[r] - first, vals[r]);
             }
-            const long long r = (long long)(c0 + lo) * P + beg[lo] + (f - pre[lo]);
-            adj_tile_add<R>(tile, keys[r] - first, vals[r]);
+        } else {
+            for (unsigned base = 0; base < total; base += RB) {
+                // stage: each thread's records, and their ranks within their cells
+                unsigned loc[RPT], rank[RPT];
+                AdjRec<R> rv[RPT];
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const unsigned f = base + (unsigned)(tid + q * T);
+                    loc[q] = kCells;
+                    rank[q] = 0u;
+                    if (f < total) {
+                        const long long r = seg_record<P>(pre, beg, nb, c0, f);
+                        loc[q] = keys[r] - first;
+                        rv[q] = vals[r];
+                        rank[q] = atomicAdd(&cnt[loc[q]], 1u);
+                    }
+                }
+                __syncthreads();
+                unsigned cc = tid < kCells ? cnt[tid] : 0u, oo;
+                Scan(scan_tmp).ExclusiveSum(cc, oo);
+                if (tid < kCells) off[tid] = oo;
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    if (loc[q] < (unsigned)kCells) {
+                        const unsigned d = off[loc[q]] + rank[q];
+                        sb[d] = rv[q].bb[0];
+                        sb[RB + d] = rv[q].bb[1];
+                        sb[2 * RB + d] = rv[q].bb[2];
+                        sb[3 * RB + d] = rv[q].w[0];
+                        sb[4 * RB + d] = rv[q].w[1];
+                        sb[5 * RB + d] = rv[q].w[2];
+                    }
+                }
+                __syncthreads();
+                // the owners sum their cell's records: corner q = (qx, qy, qz) in bits 0..2
+                const unsigned i0 = off[own], i1 = i0 + cnt[own];
+                for (unsigned i = i0; i < i1; ++i) {
+                    const float b0 = sb[i], b1 = sb[RB + i], b2 = sb[2 * RB + i];
+                    const float w0 = sb[3 * RB + i], w1 = sb[4 * RB + i], w2 = sb[5 * RB + i];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const int q = 4 * half + k;
+                        const float W = ((q & 1) ? w0 : 1.0f - w0) * (((q >> 1) & 1) ? w1 : 1.0f - w1) *
+                                        ((q >> 2) ? w2 : 1.0f - w2);
+                        acc[3 * k] += (double)(W * b0);
+                        acc[3 * k + 1] += (double)(W * b1);
+                        acc[3 * k + 2] += (double)(W * b2);
+                    }
+                }
+                __syncthreads();              // cnt / off / sb are read
+                if (tid < kCells) cnt[tid] = 0u;
+                __syncthreads();
+            }
         }
     }
-    __syncthreads();
-    for (int k = tid; k < kCells * 24; k += T) {
-        const unsigned local = (unsigned)(k / 24), comp = (unsigned)(k % 24);
-        const unsigned cell = first + local;
-        if (cell < ncell) G[(size_t)cell * 24 + comp] += tile[comp * kCells + local];
+    if constexpr (OWN) {
+        const unsigned cell = first + (unsigned)own;
+        if (cell < ncell) {
+#pragma unroll
+            for (int j = 0; j < 12; ++j) G[(size_t)cell * 24 + 12 * half + j] += acc[j];
+        }
+    } else {
+        __syncthreads();
+        for (int k = tid; k < kCells * 24; k += T) {
+            const unsigned local = (unsigned)(k / 24), comp = (unsigned)(k % 24);
+            const unsigned cell = first + local;
+            if (cell < ncell) G[(size_t)cell * 24 + comp] += tile[comp * kCells + local];
+        }
     }
 }
 
