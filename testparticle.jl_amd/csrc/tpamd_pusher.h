@@ -352,9 +352,10 @@ __device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
 // _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266, as a
 // resumable particle: init (the prologue, :160-195), step (one pass of the
 // loop, :197-248; false once the particle is done) and finish (the closing
-// record, :250-259).  run_adaptive runs it to the end; the batched kernel
-// interleaves many, handing a lane a new particle as soon as its previous one
-// is done.  Either way a particle sees the same operations in the same order.
+// record, :250-259).  The kernels interleave many particles on one wave,
+// handing a lane a new particle as soon as its previous one is done
+// (adaptive_chunk); each particle still sees the same operations in the same
+// order as when it runs alone.
 template <class S, int ALG, bool REC>
 struct AdaptiveRun {
     using R = typename S::R;
@@ -464,16 +465,6 @@ struct AdaptiveRun {
         return it - 1;
     }
 };
-
-template <class S, int ALG, bool REC>
-__device__ __forceinline__ long long run_adaptive(const KParams& p, long long i)
-{
-    AdaptiveRun<S, ALG, REC> r;
-    r.init(p, i);
-    while (r.step(p)) {
-    }
-    return r.finish(p);
-}
 
 __device__ __forceinline__ void add_steps(const KParams& p, long long steps)
 {
