@@ -264,6 +264,11 @@ def bench_dipole(args, torch, TP, W, DeviceEnsemble):
                                   "unit": "TFLOP/s", "frac": flops / peak,
                                   "kernel": "k_fixed<S64A, Boris>" if prec == "fp64"
                                   else "k_fixed<S32CA, Boris>"}}
+        if args.valu:
+            # against the FMA rate measured on this GPU (FP32 without packed math)
+            meas = args.valu["fp64_fma_flops_per_s" if prec == "fp64" else "fp32_fma_flops_per_s"]
+            out[prec]["roofline"]["measured_fma_peak"] = meas / 1e12
+            out[prec]["roofline"]["frac_of_measured_peak"] = flops / meas
         log(f"[bench] dipole {prec}: {rate:.3e} particle-steps/s ({flops / 1e12:.1f} TFLOP/s)")
         del ens
     u64, r64 = fin["fp64"]
@@ -415,19 +420,30 @@ def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
     return out
 
 
-def gather_ceiling():
-    """Measured ceiling of the random cell-record gather (the program built
-    beside the library), run as a child process.  None if it is not built."""
+def _ceiling_program(name):
+    """Runs one of the measured-ceiling programs built beside the library
+    (gather_ceiling, valu_ceiling) as a child process and returns its JSON
+    line.  None if it is not built or fails."""
     import subprocess
 
-    exe = os.path.join(ROOT, "testparticle.jl_amd", "_build", "gather_ceiling")
+    exe = os.path.join(ROOT, "testparticle.jl_amd", "_build", name)
     if not os.path.exists(exe):
         return None
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     if r.returncode != 0:
-        log(f"[bench] gather_ceiling failed: {r.stderr.strip()}")
+        log(f"[bench] {name} failed: {r.stderr.strip()}")
         return None
     return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def gather_ceiling():
+    """Measured ceiling of the random cell-record gather."""
+    return _ceiling_program("gather_ceiling")
+
+
+def valu_ceiling():
+    """Measured FMA rate of the vector ALU, FP64 and FP32 (no packed math)."""
+    return _ceiling_program("valu_ceiling")
 
 
 def copy_probe(nbytes=1 << 30, reps=3):
@@ -484,6 +500,10 @@ def main():
     if ceiling:
         log(f"[bench] measured gather ceiling: {ceiling['float_cell_gathers_per_s']:.3e} float-cell, "
             f"{ceiling['double_cell_gathers_per_s']:.3e} double-cell gathers/s")
+    args.valu = valu_ceiling() if args.ceiling and "dipole" in sections else None
+    if args.valu:
+        log(f"[bench] measured FMA rate: {args.valu['fp64_fma_flops_per_s'] / 1e12:.1f} TFLOP/s FP64, "
+            f"{args.valu['fp32_fma_flops_per_s'] / 1e12:.1f} TFLOP/s FP32")
 
     import torch
 
