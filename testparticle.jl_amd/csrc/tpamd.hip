@@ -210,6 +210,7 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
     p.static_fields = is_static(B->d) && (p.e_is_uxb || is_static(E->d)) ? 1 : 0;
     for (int d = 0; d < 3; ++d) { p.frame_origin[d] = 0.0; p.frame_period[d] = 0.0; }
     p.frame_periodic = 0;
+    p.span = 64;              // adaptive single solves set their own
     if (precision == TPAMD_FP32C) {
         const DevField* g = nullptr;
         if (is_gridded(B->d)) g = &B->d;

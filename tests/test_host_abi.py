@@ -239,3 +239,29 @@ def test_workload_generators_are_seeded():
     w = W.dipole_belt(N=64)
     r = np.linalg.norm(w.xv0[:, :3], axis=1) / TP.Re
     assert np.all((r >= 4) & (r <= 7)) and np.all(w.xv0[:, 2] == 0)
+
+
+@pytest.mark.skipif(not any(os.access(os.path.join(d, "gcc"), os.X_OK)
+                            for d in os.environ.get("PATH", "").split(os.pathsep)),
+                    reason="no C compiler")
+def test_struct_layouts_match_the_c_header(tmp_path):
+    # the ctypes mirrors against the header, compiled by the C compiler
+    import subprocess
+
+    structs = {"tpamd_solve_desc": _lib.SolveDesc, "tpamd_stats": _lib.Stats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "tpamd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for name, _ in py._fields_:
+            lines.append(f'printf("{cname} {name} %zu\\n", offsetof({cname}, {name}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {(c, n): int(v) for c, n, v in (l.split() for l in out if l)}
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for name, _ in py._fields_:
+            assert got[(cname, name)] == getattr(py, name).offset, (cname, name)
