@@ -123,6 +123,15 @@ def test_stored_E_grid_and_time_dependent_adaptive_bitwise():
         assert_bitwise(g, run_oracle(pa, alg, **kwa))
         filled = np.arange(g.u_all.shape[1])[None, :] < g.lengths[:, None]
         assert np.any(np.abs(g.u_all[filled][:, 15]) > 0)
+    # the wide-record kernels with lanes refilling from 64 x 3 particles per
+    # wave, lives cut unevenly by a sphere (121 of 128 leave, after 3 to 31
+    # records): same records
+    kwr = dict(kwa, isoutside=TP.OutsideSphere(8.0))
+    g1 = TP.solve(pa, TP.AdaptiveBoris(safety=0.05), **kwr)
+    g3 = TP.solve(pa, TP.AdaptiveBoris(safety=0.05), particles_per_lane=3, **kwr)
+    assert (g3.retcodes == 1).any() and (g3.retcodes == 0).any()
+    assert_bitwise(g3, run_oracle(pa, TP.AdaptiveBoris(safety=0.05), **kwr))
+    np.testing.assert_array_equal(g3.lengths, g1.lengths)
 
 
 def test_fp32c_records_close_to_fp64():
