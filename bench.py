@@ -486,7 +486,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the ensemble pusher runs on a single GPU "
-              "(--gpus 1); multi-GPU runs are not part of this build", file=sys.stderr)
+              "(--gpus 1); runs over several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
