@@ -354,6 +354,11 @@ int fill_params(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field*
     p.rin2_hi = (p.r_in * (1.0 + 1e-12)) * (p.r_in * (1.0 + 1e-12));
     p.rout2_lo = (p.r_out * (1.0 - 1e-12)) * (p.r_out * (1.0 - 1e-12));
     p.rout2_hi = (p.r_out * (1.0 + 1e-12)) * (p.r_out * (1.0 + 1e-12));
+    // FP32C pre-test: a float |x|^2 from the float position is within 1e-6
+    // of the exact one, so a margin of 1e-5 settles the sphere tests away
+    // from the radii (inf / NaN thresholds leave every step to the exact test)
+    p.f32.rin2_skip = std::nextafter((float)(p.rin2_hi * (1.0 + 1e-5)), std::numeric_limits<float>::infinity());
+    p.f32.rout2_skip = std::nextafter((float)(p.rout2_lo * (1.0 - 1e-5)), -std::numeric_limits<float>::infinity());
     for (int d = 0; d < 3; ++d) { p.box_lo[d] = desc->box_lo[d]; p.box_hi[d] = desc->box_hi[d]; }
     if (desc->bnd_flags & TPAMD_BND_GRID) {
         // open boundary at the faces of B's grid: the node span becomes the box

@@ -81,6 +81,10 @@ struct F32Consts {
     float P6[kRationalNMax];
     float u[3];    // UxB drift velocity
     float e0[3];   // uniform E
+    // float |x|^2 beyond which the sphere tests are decided without the
+    // double position: above rin2_skip a particle is not inside r_in, below
+    // rout2_skip it is not outside r_out
+    float rin2_skip, rout2_skip;
 };
 
 struct KParams {

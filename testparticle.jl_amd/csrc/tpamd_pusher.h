@@ -196,6 +196,18 @@ __device__ __forceinline__ bool outside(const KParams& p, const S& s, const type
 {
     const int f = p.bnd_flags;
     if (!f) return false;
+    if constexpr (sizeof(typename S::R) == 4 && S::kFK == kFkAnalytic) {
+        // FP32C without a frame, sphere tests only: decided in float away
+        // from the radii (same outcome as the exact test below)
+        if (!(f & ~(kBndSphereIn | kBndSphereOut))) {
+            float xf[3];
+            pos_f32<kFkAnalytic>(p, xn, xf);
+            const float df = (xf[0] * xf[0] + xf[1] * xf[1]) + xf[2] * xf[2];
+            const bool in_ok = !(f & kBndSphereIn) || df > p.f32.rin2_skip;
+            const bool out_ok = !(f & kBndSphereOut) || df < p.f32.rout2_skip;
+            if (in_ok && out_ok) return false;
+        }
+    }
     double x[3], v[3];
     s.abs_pos(p, xn, x);
     s.vel_d(vv, v);

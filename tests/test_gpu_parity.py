@@ -476,6 +476,26 @@ def test_parity_affine_adaptive_static_and_time_dependent(ft):
 # ---------------------------------------------------------------------------
 # FP32C: error against the FP64 path
 # ---------------------------------------------------------------------------
+def test_fp32c_sphere_tests_decided_in_float_agree_with_exact():
+    # analytic fields: FP32C settles sphere tests in float away from the
+    # radii; adding OutsideNaN (never true here) sends every step through the
+    # exact double test instead.  Loss-cone and outer-L particles cross the
+    # spheres (the oracle ends 125 / 766 / 554 of 2048 early in these cases).
+    w = W.dipole_belt(N=2048, nsteps=2000)
+    param = (w.q2m, TP.mi, TP.ZeroField(), TP.DipoleField(w.extra["b0"], w.extra["R"]), TP.ZeroField())
+    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), param)
+    for bnd in (TP.InsideSphere(3 * TP.Re), TP.OutsideSphere(6 * TP.Re),
+                TP.InsideSphere(3 * TP.Re) | TP.OutsideSphere(6.5 * TP.Re)):
+        kw = dict(dts=w.dts, nsteps=w.nsteps, savestepinterval=100, precision="fp32c")
+        a = TP.solve(prob, TP.Boris(), isoutside=bnd, **kw)
+        b = TP.solve(prob, TP.Boris(), isoutside=bnd | TP.OutsideNaN(), **kw)
+        assert (a.retcodes == 1).any() and (a.retcodes == 0).any()
+        np.testing.assert_array_equal(a.retcodes, b.retcodes)
+        np.testing.assert_array_equal(a.lengths, b.lengths)
+        np.testing.assert_array_equal(a.u_all, b.u_all)
+        np.testing.assert_array_equal(a.t_all, b.t_all)
+
+
 def test_fp32c_uniform_gyration_error():
     w = W.uniform_gyration(N=1024)
     param = (w.q2m, TP.mi, TP.ZeroField(), TP.UniformField(w.extra["B"]), TP.ZeroField())
