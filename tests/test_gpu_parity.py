@@ -342,6 +342,10 @@ def test_cell_layout_is_bit_identical(turb64, extrap):
                              (w.q2m, TP.mi, TP.UxBField(w.extra["u"], B), B, TP.ZeroField()))
         res.append(TP.solve(pr, TP.MultistepBoris(4, n=2), dt=w.dt, savestepinterval=30,
                             precision="fp32c"))
+        # FP64 over float storage, nodes and cells (one line per gather)
+        for alg in (TP.Boris(), TP.MultistepBoris(4, n=2)):
+            g = TP.solve(pr, alg, dt=w.dt, savestepinterval=30)
+            assert_bitwise(g, run_oracle(pr, alg, dt=w.dt, savestepinterval=30))
     np.testing.assert_array_equal(res[0].u_all, res[1].u_all)
 
 
