@@ -102,6 +102,19 @@ def assert_bitwise(gpu: TP.EnsembleSolution, orc: O.OracleResult, n_check=None):
                                  f"{nan_u} NaN mismatches, max |dt| {dt:.3e})")
 
 
+def assert_rows_bitwise(gpu: TP.EnsembleSolution, idx, orc: O.OracleResult):
+    """Rows idx of a device solution against an oracle run of those particles
+    alone (row k of the oracle is particle idx[k]): trajectories are
+    independent, so a sample checks a solve of any size."""
+    idx = np.asarray(idx)
+    np.testing.assert_array_equal(gpu.lengths[idx], orc.len)
+    np.testing.assert_array_equal(gpu.retcodes[idx], orc.retcode)
+    for k, i in enumerate(idx):
+        n = int(orc.len[k])
+        assert np.array_equal(gpu.u_all[i, :n], orc.u[k, :n], equal_nan=True), f"trajectory {i}: states differ"
+        assert np.array_equal(gpu.t_all[i, :n], orc.t[k, :n], equal_nan=True), f"trajectory {i}: times differ"
+
+
 def rel_err(a, b, floor=0.0):
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
 
@@ -115,4 +128,5 @@ def assert_close(gpu: TP.EnsembleSolution, orc: O.OracleResult, rtol, atol_pos=0
         np.testing.assert_allclose(gpu.u_all[i, :n, 3:], orc.u[i, :n, 3:], rtol=rtol, atol=atol_vel)
 
 
-__all__ = ["oracle_field", "run_oracle", "assert_bitwise", "assert_close", "rel_err", "_lib"]
+__all__ = ["oracle_field", "run_oracle", "assert_bitwise", "assert_rows_bitwise", "assert_close", "rel_err",
+           "_lib"]
