@@ -61,3 +61,34 @@ def test_dipole_belt_full_size():
     drift64 = np.abs(np.sum(g.u_all[ok, -1, 3:] ** 2, axis=1) - v0) / v0
     assert np.median(drift32) < 1e-5 and np.max(drift32) < 1e-3
     assert np.median(drift64) < 1e-13
+
+
+def test_batched_turbulence_full_size():
+    # 64 open turbulent boxes (32^3-128^3, float storage, cell layout), ~2e7
+    # protons, AdaptiveBoris(0.1), exits at the grid faces, <= 2,000 steps,
+    # in one persistent launch
+    inst, tspan = W.batched_turbulence(n_inst=64, total=20_000_000)
+    probs = [TP.TraceProblem(xv, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(),
+                                         TP.GridField(A, *axes, layout="cells"), TP.ZeroField()))
+             for A, axes, xv in inst]
+    assert sum(len(p.u0) for p in probs) > 19_000_000
+    alg = TP.AdaptiveBoris(safety=0.1)
+    kw = dict(isoutside=TP.OutsideGrid(), max_steps=2000, save_everystep=False)
+    sols = TP.solve_batch(probs, alg, **kw)
+    for k in (0, 21, 42, 63):
+        pr, s = probs[k], sols[k]
+        idx = np.linspace(0, len(pr.u0) - 1, 12).astype(int)
+        sub = TP.TraceProblem(pr.u0[idx], tspan, pr.p)
+        assert_rows_bitwise(s, idx, run_oracle(sub, alg, **kw))
+    # FP32C: final positions of particles that end alike (1.9e-8 median measured)
+    f = TP.solve_batch(probs, alg, precision="fp32c", **kw)
+    errs = []
+    for a, b in zip(f, sols):
+        same = (a.lengths == b.lengths) & (a.retcodes == b.retcodes)
+        last = b.lengths[same] - 1
+        xa = a.u_all[same, last, :3]
+        xb = b.u_all[same, last, :3]
+        errs.append(np.linalg.norm(xa - xb, axis=1) / np.linalg.norm(xb, axis=1))
+    e = np.concatenate(errs)
+    assert e.size > 0.9 * sum(len(p.u0) for p in probs)
+    assert np.median(e) < 1e-6
