@@ -59,6 +59,7 @@ class OrcDesc(C.Structure):
         ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
         ("dt_pp", C.POINTER(C.c_double)), ("nsteps_pp", C.c_long),
         ("save_fields", C.c_int), ("save_work", C.c_int), ("mass", C.c_double),
+        ("steps_out", C.POINTER(C.c_long)),
     ]
 
 
@@ -82,6 +83,11 @@ def lib():
         L.orc_multistep_coeffs.argtypes = [C.c_double, C.c_int, C.c_int, d3]
         L.orc_rational_polys.argtypes = [C.c_int, d3, d3, d3, d3, d3, C.POINTER(C.c_int)]
         L.orc_boris_tangent.argtypes = [C.POINTER(OrcDesc), C.POINTER(OrcField), d3, d3, d3]
+        L.orc_fdlibm_atan.argtypes = [C.c_double]
+        L.orc_fdlibm_atan.restype = C.c_double
+        L.orc_fdlibm_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_loss_tangent.argtypes = [C.POINTER(OrcDesc), C.POINTER(OrcField), C.c_long, d3, d3,
+                                       C.c_int, d3, C.POINTER(C.c_long), C.c_int, d3, d3]
         L.orc_field_eval.argtypes = [C.POINTER(OrcField), d3, C.c_double, d3]
         L.orc_field_jac.argtypes = [C.POINTER(OrcField), d3, C.c_double, d3, d3, d3]
         L.orc_work_rates.argtypes = [C.c_double, C.c_double, d3, d3, d3, d3, d3, d3]
@@ -246,12 +252,13 @@ def calculate_nout(nt, K, save_start=True, save_end=True, save_everystep=True) -
 # Solve
 # ---------------------------------------------------------------------------
 class OracleResult:
-    def __init__(self, u, t, length, retcode, elapsed):
+    def __init__(self, u, t, length, retcode, elapsed, steps=None):
         self.u = u            # (N, cap, 6)
         self.t = t            # (N, cap)
         self.len = length     # (N,)
         self.retcode = retcode
         self.elapsed = elapsed
+        self.steps = steps    # (N,) Boris steps advanced per particle
 
     def traj(self, i):
         n = min(int(self.len[i]), self.u.shape[1])
@@ -295,12 +302,14 @@ def solve(xv0, E: Field, B: Field, *, q2m, tspan, dt=0.0, alg=ALG_BORIS, adaptiv
     ln = np.zeros(N, dtype=np.int32)
     rc_arr = np.zeros(N, dtype=np.int32)
     el = np.zeros(1)
+    steps = np.zeros(N, dtype=np.int64)
+    d.steps_out = steps.ctypes.data_as(C.POINTER(C.c_long))
     rc = L.orc_solve(C.byref(d), C.byref(E.st), C.byref(B.st), _dp(xv0), N, _dp(out_u),
                      _dp(out_t), ln.ctypes.data_as(C.POINTER(C.c_int)),
                      rc_arr.ctypes.data_as(C.POINTER(C.c_int)), int(nthreads), _dp(el))
     if rc:
         raise ValueError(ERR_NAMES.get(rc, str(rc)))
-    return OracleResult(out_u, out_t, ln, rc_arr, float(el[0]))
+    return OracleResult(out_u, out_t, ln, rc_arr, float(el[0]), steps)
 
 
 def field_jac(B: Field, x, t=0.0):
@@ -342,3 +351,48 @@ def boris_tangent(xv0, B: Field, *, q2m, tspan, dt):
     if rc:
         raise ValueError(ERR_NAMES.get(rc, str(rc)))
     return xT, J.reshape(3, 6)
+
+
+TANGENT_MAX = 32
+
+
+def loss_tangent(xv0, targets, B: Field, *, q2m, tspan, dt, dense=None, flat=None, nthreads=None):
+    """Loss L = mean_i |x_i(T) - x*_i|^2 of a fixed-step standard Boris solve
+    (E = 0, periodic gridded B) and its forward-mode directional derivatives
+    along grid-value directions: dense (ndir, 3 nx ny nz) arrays in the node
+    layout (component fastest), or flat (ndir,) single values 3 node + c.
+    Returns (L, dL (ndir,))."""
+    d = OrcDesc(alg=ALG_BORIS, q2m=q2m, t0=tspan[0], t1=tspan[1], dt=dt, maxiters=1 << 40,
+                savestepinterval=1, save_start=0, save_end=1, save_everystep=0)
+    xa = np.ascontiguousarray(xv0, dtype=np.float64)
+    tg = np.ascontiguousarray(np.broadcast_to(np.asarray(targets, np.float64), (len(xa), 3)))
+    if dense is not None:
+        da = np.ascontiguousarray(dense, dtype=np.float64)
+        nd = da.shape[0]
+        dp, fp = _dp(da), None
+    else:
+        fa = np.ascontiguousarray(flat, dtype=np.int64)
+        nd = fa.shape[0]
+        dp, fp = None, fa.ctypes.data_as(C.POINTER(C.c_long))
+    if not 1 <= nd <= TANGENT_MAX:
+        raise ValueError(f"1 to {TANGENT_MAX} directions")
+    nth = nthreads or max(1, min(16, len(os.sched_getaffinity(0))))
+    loss = np.zeros(1)
+    dl = np.zeros(nd)
+    rc = lib().orc_loss_tangent(C.byref(d), C.byref(B.st), len(xa), _dp(xa), _dp(tg), nd, dp, fp,
+                                nth, _dp(loss), _dp(dl))
+    if rc:
+        raise ValueError(ERR_NAMES.get(rc, str(rc)))
+    return float(loss[0]), dl
+
+
+def fdlibm_atan(x: float) -> float:
+    """atan of the FDLIBM restatement shared with the device library."""
+    return float(lib().orc_fdlibm_atan(float(x)))
+
+
+def fdlibm_sincos(x: float):
+    """(sin x, cos x) of the FDLIBM restatement shared with the device library."""
+    s, c = C.c_double(), C.c_double()
+    lib().orc_fdlibm_sincos(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value

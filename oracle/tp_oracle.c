@@ -16,6 +16,9 @@
  * reference's known answers and against closed forms.
  */
 #include "tp_oracle.h"
+/* atan and sin/cos of the multistep reference form: the FDLIBM restatement
+ * shared with the device library, so that both round alike */
+#include "tpamd_fdlibm.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -153,10 +156,9 @@ void orc_multistep_coeffs(double tau2, int n, int form, double c[4])
                    2.0 * nn * nn);                                     /* :107 @muladd scalar */
         c[3] = (n1 + n3) / 3.0;                                        /* :108 */
     } else {
-        const double a = atan(tmag);                                   /* :110 */
+        const double a = tpf_atan(tmag);                               /* :110 */
         double s, co;
-        s = sin(nn * a);                                               /* :112 */
-        co = cos(nn * a);
+        tpf_sincos(nn * a, &s, &co);                                   /* :112 */
         const double s2 = 2.0 * s * co;                                /* :113 */
         const double c2 = fma(co, co, -(s * s));                       /* :114 @muladd scalar */
         c[0] = c2;                                                     /* :116 */
@@ -1047,6 +1049,7 @@ loop_done:;
     }
     olen[i] = (int)iout;                                        /* :545-551 */
     oret[i] = (iout < nout) ? ORC_RET_TERMINATED : ORC_RET_SUCCESS;
+    if (d->steps_out) d->steps_out[i] = final_step;
 }
 
 /* _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266 */
@@ -1126,6 +1129,7 @@ static void solve_adaptive_one(const ctx_t* c, long i, const double* xv0, double
 #undef PUSH
     olen[i] = (int)nrec;
     oret[i] = ret;
+    if (d->steps_out) d->steps_out[i] = it - 1;
 }
 
 /* ----------------------------------------------------------------------------
@@ -1239,6 +1243,236 @@ int orc_boris_tangent(const orc_desc* d, const orc_field* B, const double xv0[6]
     return ORC_OK;
 }
 
+/* ----------------------------------------------------------------------------
+ * Forward-mode derivative of the loss L = (1/N) sum_i |x_i(T) - x*_i|^2 of a
+ * fixed-step standard Boris solve (E = 0, periodic gridded B) along up to
+ * ORC_TANGENT_MAX directions in the space of grid values: the dual numbers of
+ * orc_boris_tangent, with the tangent entering through the node values
+ * instead of the initial state.  Direction k is either dense[k][3 nnodes]
+ * (node layout, component fastest) or the single value flat[k] = 3 node + c.
+ * Stands in for ForwardDiff of the reference's solve with respect to the
+ * interpolant's data (test/test_derivatives.jl:6-56 differentiates through
+ * the interpolant the same way).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    double v;
+    double d[ORC_TANGENT_MAX];
+} dualn;
+
+typedef struct {
+    int nd;
+    const double* dense;   /* [nd][3 nnodes] or NULL */
+    const int* slot;       /* [3 nnodes] direction of each value, -1 for none, or NULL */
+    long nvals;
+} tan_dirs;
+
+static inline void nc_(dualn* r, double v, int nd) { r->v = v; for (int k = 0; k < nd; ++k) r->d[k] = 0.0; }
+static inline void nadd(dualn* r, const dualn* a, const dualn* b, int nd)
+{
+    for (int k = 0; k < nd; ++k) r->d[k] = a->d[k] + b->d[k];
+    r->v = a->v + b->v;
+}
+static inline void nsub(dualn* r, const dualn* a, const dualn* b, int nd)
+{
+    for (int k = 0; k < nd; ++k) r->d[k] = a->d[k] - b->d[k];
+    r->v = a->v - b->v;
+}
+static inline void nmul(dualn* r, const dualn* a, const dualn* b, int nd)
+{
+    const double av = a->v, bv = b->v;
+    for (int k = 0; k < nd; ++k) r->d[k] = a->d[k] * bv + av * b->d[k];
+    r->v = av * bv;
+}
+static inline void ndiv(dualn* r, const dualn* a, const dualn* b, int nd)
+{
+    const double q = a->v / b->v, bv = b->v;
+    for (int k = 0; k < nd; ++k) r->d[k] = (a->d[k] - q * b->d[k]) / bv;
+    r->v = q;
+}
+static inline void nscale(dualn* r, double s, const dualn* a, int nd)
+{
+    for (int k = 0; k < nd; ++k) r->d[k] = s * a->d[k];
+    r->v = s * a->v;
+}
+/* value rounded like lerp(), fma(w, b, (1 - w) a); derivative exact */
+static inline void nlerp(dualn* r, const dualn* a, const dualn* b, const dualn* w, int nd)
+{
+    const double av = a->v, bv = b->v, wv = w->v;
+    for (int k = 0; k < nd; ++k) r->d[k] = (1.0 - wv) * a->d[k] + wv * b->d[k] + (bv - av) * w->d[k];
+    r->v = fma(wv, bv, (1.0 - wv) * av);
+}
+static inline void ncross(const dualn a[3], const dualn b[3], dualn o[3], int nd)
+{
+    dualn p, q;
+    nmul(&p, &a[1], &b[2], nd); nmul(&q, &a[2], &b[1], nd); nsub(&o[0], &p, &q, nd);
+    nmul(&p, &a[2], &b[0], nd); nmul(&q, &a[0], &b[2], nd); nsub(&o[1], &p, &q, nd);
+    nmul(&p, &a[0], &b[1], nd); nmul(&q, &a[1], &b[0], nd); nsub(&o[2], &p, &q, nd);
+}
+
+/* a node value with its tangent along the directions */
+static inline void nnode(dualn* r, const orc_field* f, const tan_dirs* td, long i, long j, long k, int c)
+{
+    const int nd = td->nd;
+    const long flat = 3 * (i + f->nx * (j + f->ny * k)) + c;
+    nc_(r, grid_node(f, i, j, k, c), nd);
+    if (td->dense) {
+        for (int q = 0; q < nd; ++q) r->d[q] = td->dense[(size_t)q * td->nvals + flat];
+    } else if (td->slot[flat] >= 0) {
+        r->d[td->slot[flat]] = 1.0;
+    }
+}
+
+static inline __attribute__((always_inline)) void ngrid(const orc_field* f, const tan_dirs* td,
+                                                         const dualn x[3], dualn out[3])
+{
+    const int nd = td->nd;
+    const long n[3] = {f->nx, f->ny, f->nz};
+    long i0[3], i1[3];
+    dualn w[3];
+    for (int d = 0; d < 3; ++d) {
+        dualn o, s;
+        nc_(&o, f->origin[d], nd);
+        nsub(&s, &x[d], &o, nd);
+        nscale(&s, 1.0 / f->spacing[d], &s, nd);
+        const double fl = floor(s.v);
+        w[d] = s;
+        w[d].v = s.v - fl;
+        long i = (long)fl % n[d];
+        if (i < 0) i += n[d];
+        i0[d] = i;
+        i1[d] = (i + 1 == n[d]) ? 0 : i + 1;   /* periodic grids only */
+    }
+    for (int c = 0; c < 3; ++c) {
+        dualn e[4], a, b, lo, hi;
+        for (int q = 0; q < 4; ++q) {
+            const long j = (q & 1) ? i1[1] : i0[1], k = (q >> 1) ? i1[2] : i0[2];
+            nnode(&a, f, td, i0[0], j, k, c);
+            nnode(&b, f, td, i1[0], j, k, c);
+            nlerp(&e[q], &a, &b, &w[0], nd);
+        }
+        nlerp(&lo, &e[0], &e[1], &w[1], nd);
+        nlerp(&hi, &e[2], &e[3], &w[1], nd);
+        nlerp(&out[c], &lo, &hi, &w[2], nd);
+    }
+}
+
+static inline __attribute__((always_inline)) void nboris(const dualn v[3], const dualn B[3], double h,
+                                                          dualn out[3], int nd)
+{
+    dualn t[3], s[3], vt[3], vp[3], vps[3], den, p, q, two_t;
+    for (int c = 0; c < 3; ++c) nscale(&t[c], h, &B[c], nd);
+    nmul(&p, &t[0], &t[0], nd);
+    nmul(&q, &t[1], &t[1], nd);
+    nadd(&p, &p, &q, nd);
+    nmul(&q, &t[2], &t[2], nd);
+    nadd(&p, &p, &q, nd);
+    nc_(&den, 1.0, nd);
+    nadd(&den, &den, &p, nd);
+    for (int c = 0; c < 3; ++c) {
+        nscale(&two_t, 2.0, &t[c], nd);
+        ndiv(&s[c], &two_t, &den, nd);
+    }
+    ncross(v, t, vt, nd);
+    for (int c = 0; c < 3; ++c) nadd(&vp[c], &v[c], &vt[c], nd);
+    ncross(vp, s, vps, nd);
+    for (int c = 0; c < 3; ++c) nadd(&out[c], &v[c], &vps[c], nd);
+}
+
+/* one particle's trajectory on duals; adds |x(T) - x*|^2 and its tangents to acc */
+static inline __attribute__((always_inline)) void tangent_particle(const orc_desc* d, const orc_field* B,
+                                                                   const tan_dirs* td, long nt,
+                                                                   const double* xv0, const double* tg,
+                                                                   int nd, double* acc)
+{
+    tan_dirs t = *td;
+    t.nd = nd;
+    const double dt = d->dt;
+    dualn x[3], v[3], Bv[3], vn[3], st;
+    for (int c = 0; c < 3; ++c) {
+        nc_(&x[c], xv0[c], nd);
+        nc_(&v[c], xv0[3 + c], nd);
+    }
+    ngrid(B, &t, x, Bv);
+    nboris(v, Bv, (d->q2m * 0.5) * (-0.5 * dt), vn, nd);             /* boris.jl:514 */
+    for (int c = 0; c < 3; ++c) v[c] = vn[c];
+    for (long it = 1; it <= nt; ++it) {                                 /* boris.jl:452-476 */
+        ngrid(B, &t, x, Bv);
+        nboris(v, Bv, (d->q2m * 0.5) * dt, vn, nd);
+        for (int c = 0; c < 3; ++c) {
+            v[c] = vn[c];
+            nscale(&st, dt, &v[c], nd);
+            nadd(&x[c], &x[c], &st, nd);
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double r = x[c].v - tg[c];
+        acc[0] += r * r;
+        for (int k = 0; k < nd; ++k) acc[1 + k] += 2.0 * r * x[c].d[k];
+    }
+}
+
+int orc_loss_tangent(const orc_desc* d, const orc_field* B, long N, const double* xv0,
+                     const double* targets, int ndir, const double* dense, const long* flat,
+                     int nthreads, double* loss, double* dloss)
+{
+    if (B->kind != ORC_FIELD_GRID || B->extrap != ORC_EXTRAP_PERIODIC) return ORC_ERR_ARG;
+    if (ndir < 1 || ndir > ORC_TANGENT_MAX || (!dense && !flat) || N < 1) return ORC_ERR_ARG;
+    long nt, nout;
+    const int e = orc_prepare_checks(d, &nt, &nout);
+    if (e != ORC_OK) return e;
+    tan_dirs td = {ndir, dense, NULL, 3 * B->nx * B->ny * B->nz};
+    int* slot = NULL;
+    if (!dense) {
+        slot = (int*)malloc(sizeof(int) * (size_t)td.nvals);
+        if (!slot) return ORC_ERR_ARG;
+        for (long q = 0; q < td.nvals; ++q) slot[q] = -1;
+        for (int k = 0; k < ndir; ++k) {
+            if (flat[k] < 0 || flat[k] >= td.nvals || slot[flat[k]] >= 0) {
+                free(slot);
+                return ORC_ERR_ARG;
+            }
+            slot[flat[k]] = k;
+        }
+        td.slot = slot;
+    }
+    if (nthreads < 1) nthreads = 1;
+    double* part = (double*)calloc((size_t)nthreads * (ORC_TANGENT_MAX + 1), sizeof(double));
+    if (!part) {
+        free(slot);
+        return ORC_ERR_ARG;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 1) num_threads(nthreads)
+#endif
+    for (int ch = 0; ch < nthreads; ++ch) {
+        const long base = N / nthreads, rem = N % nthreads;
+        const long lo = ch * base + (ch < rem ? ch : rem);
+        const long hi = lo + base + (ch < rem ? 1 : 0);
+        double* acc = part + (size_t)ch * (ORC_TANGENT_MAX + 1);
+        /* the direction count as a constant lets the compiler size the loops */
+        for (long i = lo; i < hi; ++i) {
+            switch (ndir) {
+            case 1: tangent_particle(d, B, &td, nt, xv0 + 6 * i, targets + 3 * i, 1, acc); break;
+            case 2: tangent_particle(d, B, &td, nt, xv0 + 6 * i, targets + 3 * i, 2, acc); break;
+            case 4: tangent_particle(d, B, &td, nt, xv0 + 6 * i, targets + 3 * i, 4, acc); break;
+            default: tangent_particle(d, B, &td, nt, xv0 + 6 * i, targets + 3 * i, ndir, acc); break;
+            }
+        }
+    }
+    double l = 0.0;
+    for (int k = 0; k < ndir; ++k) dloss[k] = 0.0;
+    for (int ch = 0; ch < nthreads; ++ch) {
+        const double* acc = part + (size_t)ch * (ORC_TANGENT_MAX + 1);
+        l += acc[0];
+        for (int k = 0; k < ndir; ++k) dloss[k] += acc[1 + k];
+    }
+    *loss = l / (double)N;
+    for (int k = 0; k < ndir; ++k) dloss[k] /= (double)N;
+    free(part);
+    free(slot);
+    return ORC_OK;
+}
+
 int orc_solve(const orc_desc* d, const orc_field* E, const orc_field* B,
               const double* xv0, long N, double* out_u, double* out_t,
               int* out_len, int* retcode, int nthreads, double* elapsed_s)
@@ -1284,3 +1518,7 @@ int orc_solve(const orc_desc* d, const orc_field* E, const orc_field* B,
 #endif
     return ORC_OK;
 }
+
+/* the shared FDLIBM functions, exported for their accuracy tests */
+double orc_fdlibm_atan(double x) { return tpf_atan(x); }
+void orc_fdlibm_sincos(double x, double* s, double* c) { tpf_sincos(x, s, c); }

@@ -100,6 +100,7 @@ typedef struct orc_desc {
     int save_fields;       /* records gain E, B (boris.jl:405-440) */
     int save_work;         /* records gain P_par, P_fermi, P_grad, P_betatron */
     double mass;           /* p[2], for the work rates */
+    long* steps_out;       /* optional [N]: Boris steps each particle advanced */
 } orc_desc;
 
 /* record columns: 6 + 6 (save_fields) + 4 (save_work) */
@@ -142,6 +143,19 @@ int orc_prepare_checks(const orc_desc* d, long* nt, long* nout);
  * periodic gridded B, by forward-mode dual numbers: xT[3], J[3][6] row-major */
 int orc_boris_tangent(const orc_desc* d, const orc_field* B, const double xv0[6], double xT[3],
                       double J[18]);
+/* Forward-mode directional derivatives of L = (1/N) sum_i |x_i(T) - x*_i|^2
+ * (same solve) along ndir <= ORC_TANGENT_MAX directions in the grid values:
+ * dense[ndir][3 nx ny nz] (node layout, component fastest) when dense is not
+ * NULL, else the single values flat[ndir] = 3 node + c (distinct).  Particles
+ * are split into nthreads contiguous chunks; *loss = L, dloss[k] = dL . dir_k. */
+#define ORC_TANGENT_MAX 32
+/* atan and sin/cos of the multistep reference form (FDLIBM, shared with the
+ * device library's tpamd_fdlibm.h) */
+double orc_fdlibm_atan(double x);
+void orc_fdlibm_sincos(double x, double* s, double* c);
+int orc_loss_tangent(const orc_desc* d, const orc_field* B, long N, const double* xv0,
+                     const double* targets, int ndir, const double* dense, const long* flat,
+                     int nthreads, double* loss, double* dloss);
 int orc_solve(const orc_desc* d, const orc_field* E, const orc_field* B,
               const double* xv0, long N, double* out_u, double* out_t,
               int* out_len, int* retcode, int nthreads, double* elapsed_s);

@@ -131,6 +131,10 @@ int check_desc(tpamd_ctx* ctx, const tpamd_solve_desc* d, int64_t* nt, int64_t* 
         if (d->form == TPAMD_FORM_RATIONAL && d->nsub > kRationalNMax)
             return fail(ctx, TPAMD_ERR_UNSUPPORTED, "rational multistep form supports n <= %d",
                         kRationalNMax);
+        // the reference form's sincos(n atan(tau)) reduces n atan(tau) < n pi/2
+        // with the medium-size FDLIBM reduction, which holds below 2^20 pi/2
+        if (d->form != TPAMD_FORM_RATIONAL && d->nsub > (1 << 18))
+            return fail(ctx, TPAMD_ERR_UNSUPPORTED, "reference multistep form supports n <= 262144");
     }
     if (d->precision != TPAMD_FP64 && d->precision != TPAMD_FP32C)
         return fail(ctx, TPAMD_ERR_ARGUMENT, "unknown precision %d", d->precision);
@@ -238,7 +242,8 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
 // adaptive) or the rational multistep update with n <= 2 (fixed step), FP32C
 // on float storage.
 // ---------------------------------------------------------------------------
-// -1 when no specialisation applies, else the subcycle class NS (0 = Boris)
+// -1 when no specialisation applies, else the subcycle class NS (0 = Boris,
+// n = rational multistep, kRefBase + n = reference-form multistep)
 int spec_class(const tpamd_solve_desc* d, const KParams& p, int* emode)
 {
     if (d->flags & TPAMD_FLAG_GENERIC_KERNELS) return -1;
@@ -249,7 +254,7 @@ int spec_class(const tpamd_solve_desc* d, const KParams& p, int* emode)
     else return -1;
     int ns;
     if (d->alg == TPAMD_ALG_BORIS) ns = 0;
-    else if (d->form == TPAMD_FORM_RATIONAL && (d->nsub == 1 || d->nsub == 2)) ns = d->nsub;
+    else if (d->nsub == 1 || d->nsub == 2) ns = d->form == TPAMD_FORM_RATIONAL ? d->nsub : kRefBase + d->nsub;
     else return -1;
     if (d->adaptive && ns != 0) return -1;
     const double es = p.B.dtype == 1 ? 4.0 : 8.0;
@@ -267,8 +272,14 @@ void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec,
     if (ns >= 0) {
         const bool per = p.B.extrap == kExtrapPeriodic;
         const bool cells = p.B.cells != nullptr;
-        if (d->precision == TPAMD_FP32C) launch_spec32(r, emode, ns, per, cells);
-        else launch_spec64(r, emode, ns, per, p.B.dtype == 1, cells);
+        if (ns > kRefBase) {
+            if (d->precision == TPAMD_FP32C) launch_spec32r(r, emode, ns, per, cells);
+            else launch_spec64r(r, emode, ns, per, p.B.dtype == 1, cells);
+        } else if (d->precision == TPAMD_FP32C) {
+            launch_spec32(r, emode, ns, per, cells);
+        } else {
+            launch_spec64(r, emode, ns, per, p.B.dtype == 1, cells);
+        }
         return;
     }
     if (d->precision == TPAMD_FP64) {

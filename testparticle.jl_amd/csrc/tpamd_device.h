@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tpamd_fdlibm.h"
+
 namespace tpamd {
 
 constexpr int kRationalNMax = 12;
@@ -223,9 +225,10 @@ __device__ __forceinline__ void multistep_f64(const KParams& p, const double v[3
             c3 = fma(-((4.0 * nn * nn + 2.0 * nn * nn * nn * nn) / 3.0), tau2, 2.0 * nn * nn);
             c6 = (n1 + n3) / 3.0;
         } else {
-            const double a = atan(tmag);
+            // FDLIBM atan and sin/cos (tpamd_fdlibm.h), rounding as the oracle does
+            const double a = tpf_atan(tmag);
             double s, co;
-            sincos(nn * a, &s, &co);
+            tpf_sincos(nn * a, &s, &co);
             const double s2 = 2.0 * s * co;
             c1 = fma(co, co, -(s * s));
             c2 = s2 / tmag;

@@ -220,11 +220,43 @@ __device__ __forceinline__ double horner_fixed(const double* p, double u)
     return r;
 }
 
+// Subcycle classes of the specialised multistep kernels: NS = n for the
+// rational coefficients, NS = kRefBase + n for the reference form
+// (multistep_boris.jl:94-120: atan, sincos and the Taylor branch).
+constexpr int kRefBase = 100;
+__host__ __device__ constexpr int ns_sub(int NS) { return NS > kRefBase ? NS - kRefBase : NS; }
+
+// reference-form coefficients c1, c2, c3, c6 (multistep_boris.jl:94-120),
+// with the FDLIBM atan and sin/cos the oracle uses
+__device__ __forceinline__ void multistep_ref_coeffs_f64(double tau2, int n, double& c1, double& c2,
+                                                         double& c3, double& c6)
+{
+    const double tmag = sqrt(tau2);
+    const double nn = (double)n;
+    if (tmag < 1.0e-4) {
+        c1 = fma(-(2.0 * nn * nn), tau2, 1.0);
+        const double n1 = 2.0 * nn, n3 = 4.0 * nn * nn * nn;
+        c2 = fma(-((n1 + n3) / 3.0), tau2, n1);
+        c3 = fma(-((4.0 * nn * nn + 2.0 * nn * nn * nn * nn) / 3.0), tau2, 2.0 * nn * nn);
+        c6 = (n1 + n3) / 3.0;
+    } else {
+        const double a = tpf_atan(tmag);
+        double s, co;
+        tpf_sincos(nn * a, &s, &co);
+        const double s2 = 2.0 * s * co;
+        c1 = fma(co, co, -(s * s));
+        c2 = s2 / tmag;
+        c3 = 2.0 * s * s / tau2;
+        c6 = (2.0 * nn - c2) / tau2;
+    }
+}
+
 template <int NS>
 __device__ __forceinline__ void multistep_f64_n(const KParams& p, const double v[3], const double E[3],
                                                 const double B[3], double dt, double out[3])
 {
-    const double factor = p.q2m * dt / (2.0 * NS);
+    constexpr int NSUB = ns_sub(NS);
+    const double factor = p.q2m * dt / (2.0 * NSUB);
     double tn[3], en[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) tn[c] = factor * B[c];
@@ -247,11 +279,16 @@ __device__ __forceinline__ void multistep_f64_n(const KParams& p, const double v
         for (int c = 0; c < 3; ++c) tn[c] = fN * tn[c];
     }
     const double tau2 = dot3(tn, tn);
-    const double rD = 1.0 / horner_fixed<NS>(p.poly.D, tau2);
-    const double c1 = horner_fixed<NS>(p.poly.P1, tau2) * rD;
-    const double c2 = horner_fixed<NS - 1>(p.poly.P2, tau2) * rD;
-    const double c3 = horner_fixed<NS - 1>(p.poly.P3, tau2) * rD;
-    const double c6 = horner_fixed<NS - 1>(p.poly.P6, tau2) * rD;
+    double c1, c2, c3, c6;
+    if constexpr (NS > kRefBase) {
+        multistep_ref_coeffs_f64(tau2, NSUB, c1, c2, c3, c6);
+    } else {
+        const double rD = 1.0 / horner_fixed<NSUB>(p.poly.D, tau2);
+        c1 = horner_fixed<NSUB>(p.poly.P1, tau2) * rD;
+        c2 = horner_fixed<NSUB - 1>(p.poly.P2, tau2) * rD;
+        c3 = horner_fixed<NSUB - 1>(p.poly.P3, tau2) * rD;
+        c6 = horner_fixed<NSUB - 1>(p.poly.P6, tau2) * rD;
+    }
     const double vdt = dot3(v, tn), edt = dot3(en, tn);
     double vxt[3], ext[3];
     cross3(v, tn, vxt);
@@ -307,11 +344,31 @@ __device__ __forceinline__ void multistep_f32_n(const KParams& p, const float v[
         for (int c = 0; c < 3; ++c) tn[c] = fN * tn[c];
     }
     const float tau2 = dot3f(tn, tn);
-    const float rD = __builtin_amdgcn_rcpf(horner_fixed_f<NS>(p.f32.D, tau2));
-    const float c1 = horner_fixed_f<NS>(p.f32.P1, tau2) * rD;
-    const float c2 = horner_fixed_f<NS - 1>(p.f32.P2, tau2) * rD;
-    const float c3 = horner_fixed_f<NS - 1>(p.f32.P3, tau2) * rD;
-    const float c6 = horner_fixed_f<NS - 1>(p.f32.P6, tau2) * rD;
+    float c1, c2, c3, c6;
+    if constexpr (NS > kRefBase) {
+        // reference form in float (multistep_boris.jl:94-120)
+        constexpr float nn = (float)ns_sub(NS);
+        const float tmag = sqrtf(tau2);
+        if (tmag < 1.0e-4f) {
+            c1 = fmaf(-(2.0f * nn * nn), tau2, 1.0f);
+            c2 = fmaf(-((2.0f * nn + 4.0f * nn * nn * nn) / 3.0f), tau2, 2.0f * nn);
+            c3 = fmaf(-((4.0f * nn * nn + 2.0f * nn * nn * nn * nn) / 3.0f), tau2, 2.0f * nn * nn);
+            c6 = (2.0f * nn + 4.0f * nn * nn * nn) / 3.0f;
+        } else {
+            float s, co;
+            sincosf(nn * atanf(tmag), &s, &co);
+            c1 = fmaf(co, co, -(s * s));
+            c2 = 2.0f * s * co / tmag;
+            c3 = 2.0f * s * s / tau2;
+            c6 = (2.0f * nn - c2) / tau2;
+        }
+    } else {
+        const float rD = __builtin_amdgcn_rcpf(horner_fixed_f<NS>(p.f32.D, tau2));
+        c1 = horner_fixed_f<NS>(p.f32.P1, tau2) * rD;
+        c2 = horner_fixed_f<NS - 1>(p.f32.P2, tau2) * rD;
+        c3 = horner_fixed_f<NS - 1>(p.f32.P3, tau2) * rD;
+        c6 = horner_fixed_f<NS - 1>(p.f32.P6, tau2) * rD;
+    }
     const float vdt = dot3f(v, tn), edt = dot3f(en, tn);
     float vxt[3], ext[3];
     cross3f(v, tn, vxt);

@@ -108,6 +108,9 @@ void launch_generic32_mesh(const LaunchReq& r, int alg);
 // cells the cell-replicated layout
 void launch_spec64(const LaunchReq& r, int emode, int ns, bool per, bool f32, bool cells);
 void launch_spec32(const LaunchReq& r, int emode, int ns, bool per, bool cells);
+// the reference-form multistep classes (ns = kRefBase + n)
+void launch_spec64r(const LaunchReq& r, int emode, int ns, bool per, bool f32, bool cells);
+void launch_spec32r(const LaunchReq& r, int emode, int ns, bool per, bool cells);
 // the checkpointed adjoint (tpamd_adj.hip)
 int adjoint_launch(tpamd_ctx* ctx, const KParams& p, const tpamd_field* B, const double* xv0,
                    int64_t N, const double* targets, int K, int64_t nt, double* loss,

@@ -549,17 +549,30 @@ __global__ void __launch_bounds__(256) k_adaptive(const KParams p)
 
 // ---------------------------------------------------------------------------
 // Specialised gridded-B policies (see tpamd_grid.h).  NS = 0 is the Boris
-// update, NS >= 1 the rational multistep update with n = NS subcycles.
+// update, 1 <= NS < kRefBase the rational multistep update with n = NS
+// subcycles, NS = kRefBase + n the reference-form multistep update.
 // ---------------------------------------------------------------------------
 template <int EMODE, int NS, bool PER, typename T, bool CELLS>
 struct S64G : S64 {
     static constexpr int kNS = NS;
-    // FP64 periodic cells with E = -u x B: bounding the kernel to 96 VGPRs
-    // (5 waves per SIMD instead of 4) hides more of the gather latency and
-    // beats the 44-56 B it spills (+10 % Boris, +4 % multistep at 256^3).
-    // With a stored uniform E the same bound spills 100-136 B and loses 20 %,
-    // and the non-periodic variants would spill 76-84 B: no bound there.
-    static constexpr int kWaves = (EMODE == kEUxB && PER && CELLS && sizeof(T) == 8) ? 5 : 1;
+    // FP64 periodic cells with E = -u x B: an occupancy target for the
+    // compiler (amdgpu_waves_per_eu).  Measured at 256^3, MultistepBoris4(n=2):
+    // the rational form at 5 waves spills 56 B per lane (2.36e10
+    // particle-steps/s), at 4 waves it fits 120 VGPRs with no spill (2.58e10);
+    // with no target the same 120 VGPRs ran at 2.27e10.  The reference form
+    // (atan, sincos, three divisions) holds more live values: at 5 waves it
+    // spilled 200 B (1.80e10), at 4 waves 96 B (2.12e10), at 3 it takes 163
+    // VGPRs and no spill (2.61e10); with no target, 2.27e10.  With a stored
+    // uniform E a 5-wave bound spilled 100-136 B and lost 20 %, and the
+    // non-periodic variants would spill 76-84 B: no target there.
+#ifndef TPAMD_RAT_WAVES
+#define TPAMD_RAT_WAVES 4
+#endif
+#ifndef TPAMD_REF_WAVES
+#define TPAMD_REF_WAVES 3
+#endif
+    static constexpr int kWaves = (EMODE == kEUxB && PER && CELLS && sizeof(T) == 8)
+                                      ? (NS < kRefBase ? TPAMD_RAT_WAVES : TPAMD_REF_WAVES) : 1;
     __device__ __forceinline__ void bfield(const KParams& p, double B[3]) const
     {
         int i0[3], i1[3];
@@ -721,6 +734,21 @@ void sel_g64_ns(const L& l, int ns, bool per)
     if (ns == 0) sel_g64<L, EMODE, 0, T, CELLS>(l, per);
     else if (ns == 1) sel_g64<L, EMODE, 1, T, CELLS>(l, per);
     else sel_g64<L, EMODE, 2, T, CELLS>(l, per);
+}
+
+// the reference-form classes kRefBase + 1 and kRefBase + 2
+template <class L, int EMODE, bool CELLS>
+void sel_g32_nsr(const L& l, int ns, bool per)
+{
+    if (ns == kRefBase + 1) sel_g32<L, EMODE, kRefBase + 1, CELLS>(l, per);
+    else sel_g32<L, EMODE, kRefBase + 2, CELLS>(l, per);
+}
+
+template <class L, int EMODE, typename T, bool CELLS>
+void sel_g64_nsr(const L& l, int ns, bool per)
+{
+    if (ns == kRefBase + 1) sel_g64<L, EMODE, kRefBase + 1, T, CELLS>(l, per);
+    else sel_g64<L, EMODE, kRefBase + 2, T, CELLS>(l, per);
 }
 
 // one solve: a thread per particle

@@ -65,7 +65,7 @@ def initial_states(prob: TraceProblem, trajectories: int, seed=None) -> np.ndarr
 
 def make_desc(prob: TraceProblem, alg: AbstractBoris, *, dt=None, savestepinterval=1,
               isoutside=None, save_start=True, save_end=True, save_everystep=True,
-              maxiters=1_000_000, precision="fp64", form="rational", max_steps=0,
+              maxiters=1_000_000, precision="fp64", form="reference", max_steps=0,
               nout_cap=0, dts=None, nsteps=0, workgroup_size=0,
               kernels="auto", save_fields=False, save_work=False,
               particles_per_lane=0) -> _lib.SolveDesc:
@@ -156,13 +156,16 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
           savestepinterval: int = 1, dt=None, isoutside=None, save_start: bool = True,
           save_end: bool = True, save_everystep: bool = True, save_fields: bool = False,
           save_work: bool = False, maxiters: int = 1_000_000, batch_size=None, seed=None,
-          workgroup_size: int = 0, precision: str = "fp64", form: str = "rational",
+          workgroup_size: int = 0, precision: str = "fp64", form: str = "reference",
           max_steps: int = 0, nout_cap=None, dts=None, nsteps: int = 0,
           kernels: str = "auto", particles_per_lane: int = 0, device=None) -> EnsembleSolution:
     """Trace an ensemble with the Boris family on the GPU.
 
     Extensions beyond the reference's keywords: `precision` ("fp64" |
-    "fp32c"), `form` ("rational" | "reference", multistep coefficients),
+    "fp32c"), `form` (multistep coefficients: "reference", the default, as
+    multistep_boris.jl:94-120 computes them, with atan and sin/cos from
+    FDLIBM, bit-identical to the oracle; "rational", the same coefficients as
+    ratios of integer polynomials in tau^2, no transcendental),
     `max_steps` (adaptive step cap), `dts` + `nsteps` (per-particle step,
     fixed-step only), `nout_cap` (adaptive record capacity), `kernels`
     ("auto" | "generic": skip the gridded-field specialisations),
@@ -230,7 +233,7 @@ def _adaptive_cap(save_start, save_everystep, max_steps, savestepinterval):
 def solve_batch(problems, alg: AbstractBoris, *, dt=None, savestepinterval: int = 1,
                 isoutside=None, save_start: bool = True, save_end: bool = True,
                 save_everystep: bool = True, maxiters: int = 1_000_000,
-                precision: str = "fp64", form: str = "rational", max_steps: int = 0,
+                precision: str = "fp64", form: str = "reference", max_steps: int = 0,
                 nout_cap=None, kernels: str = "auto", device: int = 0,
                 save_fields: bool = False, save_work: bool = False):
     """Solve many problems, each with its own fields and (N_k, 6) initial

@@ -41,7 +41,7 @@ def oracle_field(f, B_oracle=None):
 
 
 def run_oracle(prob, alg, *, dt=None, savestepinterval=1, isoutside=None, save_start=True,
-               save_end=True, save_everystep=True, maxiters=1_000_000, form="rational",
+               save_end=True, save_everystep=True, maxiters=1_000_000, form="reference",
                max_steps=0, nout_cap=None, dts=None, nsteps=0, nthreads=8, trajectories=1,
                save_fields=False, save_work=False):
     xv0 = TP.initial_states(prob, trajectories)     # host-side prob_func, no device

@@ -70,9 +70,14 @@ def test_argument_errors(kats):
     d.order_N = 3
     with pytest.raises(TP.ArgumentError, match="order N"):
         TP.plan(d)
-    d = _desc(dt=1e-9, alg=TP.MultistepBoris(4, n=13))
+    # the rational coefficients are tabulated up to n = 12; the reference form
+    # (the default) takes any n up to 2^18 (its sincos argument reduction)
+    d = _desc(dt=1e-9, alg=TP.MultistepBoris(4, n=13), form="rational")
     with pytest.raises(NotImplementedError):
         TP.plan(d)
+    TP.plan(_desc(dt=1e-9, alg=TP.MultistepBoris(4, n=13)))
+    with pytest.raises(NotImplementedError):
+        TP.plan(_desc(dt=1e-9, alg=TP.MultistepBoris(4, n=(1 << 18) + 1)))
     with pytest.raises(TypeError, match="dt"):
         _desc()
 

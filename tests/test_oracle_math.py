@@ -273,3 +273,134 @@ def test_boris_tangent_matches_solve_and_finite_differences(orc):
     np.testing.assert_allclose(J, fd, rtol=0, atol=1e-6 * np.abs(fd).max())
     # a dropped field-gradient term would make the position block the identity
     assert np.abs(J[:, :3] - np.eye(3)).max() > 1e-3
+
+
+def test_loss_tangent_closed_form_in_uniform_field(orc):
+    # In a uniform B = B0 z every Boris step is an exact rotation about z by
+    # -theta, theta = 2 atan(h B0) (h = q2m dt / 2), and the back half-step at
+    # t0 (boris.jl:514) rotates by +phi, phi = 2 atan(h B0 / 2).  So
+    #   x(T) = x0 + dt sum_{n=1..nt} R(phi - n theta) v0,
+    # and along the direction "every node's z component + 1" (d/dB0)
+    #   dx(T)/dB0 = dt sum_n (phi' - n theta') z x R(phi - n theta) v0.
+    q2m, dt, nt, B0 = 1.0, 0.07, 150, 1.3
+    rng = np.random.default_rng(21)
+    N = 5
+    xv0 = np.hstack([rng.uniform(0, 3, (N, 3)), rng.normal(size=(N, 3))])
+    tg = rng.uniform(0, 3, (N, 3))
+    n = (4, 3, 5)
+    data = np.zeros((n[2], n[1], n[0], 3))
+    data[..., 2] = B0
+    g = orc.grid(data, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), orc.EXTRAP_PERIODIC)
+    dz = np.zeros((1, data.size))
+    dz[0, 2::3] = 1.0
+    L, dL = orc.loss_tangent(xv0, tg, g, q2m=q2m, tspan=(0.0, nt * dt), dt=dt, dense=dz, nthreads=2)
+    h = q2m * dt / 2
+    th, ph = 2 * np.arctan(h * B0), 2 * np.arctan(h * B0 / 2)
+    dth, dph = 2 * h / (1 + (h * B0) ** 2), h / (1 + (h * B0 / 2) ** 2)
+    a = ph - np.arange(1, nt + 1) * th                      # rotation angle of v_{n-1/2}
+    da = dph - np.arange(1, nt + 1) * dth
+    ca, sa = np.cos(a), np.sin(a)
+    xT = np.empty((N, 3))
+    dxT = np.zeros((N, 3))
+    for i in range(N):
+        vx, vy, vz = xv0[i, 3:]
+        rx, ry = ca * vx - sa * vy, sa * vx + ca * vy       # R(a) v0 (counter-clockwise)
+        xT[i] = xv0[i, :3] + dt * np.array([rx.sum(), ry.sum(), nt * vz])
+        dxT[i, :2] = dt * np.array([np.sum(-da * ry), np.sum(da * rx)])   # z x (R v0)
+    L_ref = np.mean(np.sum((xT - tg) ** 2, axis=1))
+    dL_ref = np.mean(np.sum(2 * (xT - tg) * dxT, axis=1))
+    assert L == pytest.approx(L_ref, rel=1e-11)
+    assert dL[0] == pytest.approx(dL_ref, rel=1e-9)
+
+
+def test_loss_tangent_gridded_field(orc):
+    # periodic random grid: the directional derivatives are linear in the
+    # direction, a single-value direction equals its dense one-hot bit for bit,
+    # the primal loss is the solver's, and central differences of the solver's
+    # loss (per-particle differences, h = 1e-7) agree
+    q2m, dt, nt = 1.0, 0.05, 60
+    n, Lbox = (6, 5, 7), 4.0
+    sp = tuple(Lbox / n[d] for d in range(3))
+    data = RNG.normal(scale=0.4, size=(n[2], n[1], n[0], 3)) + np.array([0.0, 0.0, 1.0])
+    rng = np.random.default_rng(22)
+    N = 40
+    xv0 = np.hstack([rng.uniform(0, Lbox, (N, 3)), rng.normal(scale=0.5, size=(N, 3))])
+    tg = rng.uniform(0, Lbox, (N, 3))
+    tspan = (0.0, nt * dt)
+    g = orc.grid(data, (0.0, 0.0, 0.0), sp, orc.EXTRAP_PERIODIC)
+    d1, d2 = rng.normal(size=(2, data.size))
+    L, dL = orc.loss_tangent(xv0, tg, g, q2m=q2m, tspan=tspan, dt=dt,
+                             dense=np.stack([d1, d2, 0.5 * d1 - 2.0 * d2]), nthreads=3)
+    assert dL[2] == pytest.approx(0.5 * dL[0] - 2.0 * dL[1], rel=1e-10)
+    r = orc.solve(xv0, orc.zero(), g, q2m=q2m, tspan=tspan, dt=dt, save_everystep=False,
+                  save_start=False)
+    xT = np.array([r.traj(i)[1][-1][:3] for i in range(N)])
+    assert L == pytest.approx(np.mean(np.sum((xT - tg) ** 2, axis=1)), rel=1e-14)
+
+    def final(dat):
+        gg = orc.grid(dat, (0.0, 0.0, 0.0), sp, orc.EXTRAP_PERIODIC)
+        rr = orc.solve(xv0, orc.zero(), gg, q2m=q2m, tspan=tspan, dt=dt, save_everystep=False,
+                       save_start=False)
+        return np.array([rr.traj(i)[1][-1][:3] for i in range(N)])
+
+    eps = 1e-7
+    for k, dvec in enumerate((d1, d2)):
+        xp = final(data + eps * dvec.reshape(data.shape))
+        xm = final(data - eps * dvec.reshape(data.shape))
+        fd = np.mean(np.sum((xp - xm) * (xp + xm - 2 * tg), axis=1)) / (2 * eps)
+        assert dL[k] == pytest.approx(fd, rel=1e-6), k
+    flat = np.array([5, 3 * 17 + 2, 3 * 100 + 1])
+    Ls, dLs = orc.loss_tangent(xv0, tg, g, q2m=q2m, tspan=tspan, dt=dt, flat=flat, nthreads=3)
+    onehot = np.zeros((3, data.size))
+    onehot[np.arange(3), flat] = 1.0
+    Ld, dLd = orc.loss_tangent(xv0, tg, g, q2m=q2m, tspan=tspan, dt=dt, dense=onehot, nthreads=3)
+    assert Ls == Ld
+    np.testing.assert_array_equal(dLs, dLd)
+    assert np.all(dLs != 0.0)
+
+
+def _ulp_err(got, exact):
+    import math
+
+    import mpmath
+    e = float(abs(mpmath.mpf(got) - exact))
+    return e / math.ulp(float(exact))
+
+
+def test_fdlibm_atan_within_one_ulp(orc):
+    # the reference form's atan (src/boris/multistep_boris.jl:110), as the
+    # device computes it: within 1 ulp of atan in 60-digit arithmetic on every
+    # reduction interval (|x| < 7/16, 7/16-11/16, 11/16-19/16, 19/16-39/16,
+    # above), and exact at the special values
+    import mpmath
+    mpmath.mp.dps = 60
+    rng = np.random.default_rng(31)
+    xs = np.concatenate([rng.uniform(1e-9, 0.4375, 3000), rng.uniform(0.4375, 2.4375, 3000),
+                         10 ** rng.uniform(0.39, 19.8, 2000), -rng.uniform(0, 3, 1000),
+                         [0.4375, 0.6875, 1.1875, 2.4375, 1e-4, 0.039, 7.450580596923828e-9]])
+    worst = max(_ulp_err(orc.fdlibm_atan(float(x)), mpmath.atan(mpmath.mpf(float(x)))) for x in xs)
+    assert worst < 1.0, worst
+    assert orc.fdlibm_atan(0.0) == 0.0 and np.signbit(orc.fdlibm_atan(-0.0))
+    assert orc.fdlibm_atan(np.inf) == orc.fdlibm_atan(1e30) == np.pi / 2
+    assert np.isnan(orc.fdlibm_atan(np.nan))
+
+
+def test_fdlibm_sincos_within_one_ulp(orc):
+    # sin and cos of the reference form (multistep_boris.jl:112) with the
+    # medium-size argument reduction: within 1 ulp of the 60-digit values up
+    # to |x| ~ 2^19 pi/2, including arguments next to multiples of pi/2
+    import mpmath
+    mpmath.mp.dps = 60
+    rng = np.random.default_rng(32)
+    near = np.array([k * np.pi / 2 for k in range(1, 400)])
+    xs = np.concatenate([rng.uniform(-0.8, 0.8, 2000), rng.uniform(-40, 40, 2000),
+                         rng.uniform(-8.2e5, 8.2e5, 1000), near, np.nextafter(near, 0), 1e-9 * near])
+    ws = wc = 0.0
+    for x in xs:
+        s, c = orc.fdlibm_sincos(float(x))
+        xm = mpmath.mpf(float(x))
+        ws = max(ws, _ulp_err(s, mpmath.sin(xm)))
+        wc = max(wc, _ulp_err(c, mpmath.cos(xm)))
+    assert ws < 1.0 and wc < 1.0, (ws, wc)
+    s, c = orc.fdlibm_sincos(2.0e6)          # past the medium reduction: NaN, never asked for
+    assert np.isnan(s) and np.isnan(c)
