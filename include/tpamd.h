@@ -41,7 +41,9 @@ typedef enum tpamd_status {
     TPAMD_ERR_DEVICE = 5,        /* HIP runtime error */
     TPAMD_ERR_OOM = 6,           /* device allocation failed */
     TPAMD_ERR_UNSUPPORTED = 7,   /* combination the device path does not provide */
-    TPAMD_ERR_CAPACITY = 8       /* adaptive records exceeded nout_cap (stats->max_len says how many) */
+    TPAMD_ERR_CAPACITY = 8,      /* adaptive records exceeded nout_cap (stats->max_len says how many) */
+    TPAMD_ERR_NONFINITE = 9      /* tpamd_adjoint: a loss or gradient entry is not finite (outputs are
+                                    still written, so the caller can see which) */
 } tpamd_status;
 
 /* Field kinds: the device-side menu that replaces the Julia field callables
@@ -275,8 +277,12 @@ int32_t tpamd_solve_device(tpamd_ctx* ctx, const tpamd_solve_desc* desc,
  *   desc: alg = BORIS, adaptive = 0, precision FP64 or FP32C, no boundary.
  *   ckpt_interval K: FP64 state checkpoints every K steps; the reverse sweep
  *     recomputes each K-step segment.
- *   targets[N][3]; dL_dB in B's node layout [nz][ny][nx][3] (accumulated in
- *   float); dL_dxv0[N][6].  stats->kernel_ms = forward pass, aux_ms = reverse. */
+ *   targets[N][3]; dL_dB in B's node layout [nz][ny][nx][3] (summed in double);
+ *   dL_dxv0[N][6].  stats->kernel_ms = forward pass, aux_ms = reverse.
+ *   FP32C: forward, recompute and reverse steps in float; the adjoint state
+ *   carries a per-particle power-of-two exponent, so gradients far beyond the
+ *   float range (a chaotic flow's grow like exp(lambda T)) stay finite.
+ *   Returns TPAMD_ERR_NONFINITE when any output entry is not finite. */
 int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* B,
                       const double* xv0, int64_t N, const double* targets,
                       int32_t ckpt_interval, double* loss, double* dL_dB, double* dL_dxv0,

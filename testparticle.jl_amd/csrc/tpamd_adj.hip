@@ -83,7 +83,28 @@ __device__ __forceinline__ void adj_zero_record(const AdjParams& a, size_t rec)
         v.bb[k] = (R)0;
         v.w[k] = (R)0;
     }
+    if constexpr (sizeof(R) == 4) v.e = 0;
     ((AdjRec<R>*)a.rec_val)[rec] = v;
+}
+
+// Float adjoint state: (xbar, ubar) = (xb, ub) * 2^ex.  Rescales the
+// mantissas so that the largest component lies in [0.5, 1) (exact powers of
+// two; a zero or non-finite state is left as it is, and a non-finite one
+// reaches the outputs, whose scan reports it).
+__device__ __forceinline__ void adj_renorm(float xb[3], float ub[3], int& ex)
+{
+    const float m = fmaxf(fmaxf(fmaxf(fabsf(xb[0]), fabsf(xb[1])), fmaxf(fabsf(xb[2]), fabsf(ub[0]))),
+                          fmaxf(fabsf(ub[1]), fabsf(ub[2])));
+    if (m > 0.0f && m <= 3.4028235e38f) {
+        int k;
+        (void)frexpf(m, &k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xb[c] = ldexpf(xb[c], -k);
+            ub[c] = ldexpf(ub[c], -k);
+        }
+        ex += k;
+    }
 }
 
 // one reverse step at the state held in s: (xbar, ubar) <- VJP of the step
@@ -93,8 +114,8 @@ template <bool F32, bool PER, typename T, bool CELLS>
 __device__ __forceinline__ void adj_step_back(const KParams& p, const AdjParams& a,
                                               const AdjState<F32, PER, T, CELLS>& s, const typename AdjState<F32, PER, T, CELLS>::R u[3],
                                               double h, double dt, typename AdjState<F32, PER, T, CELLS>::R xb[3],
-                                              typename AdjState<F32, PER, T, CELLS>::R ub[3], bool with_dt,
-                                              size_t rec)
+                                              typename AdjState<F32, PER, T, CELLS>::R ub[3], int& ex,
+                                              bool with_dt, size_t rec)
 {
     using R = typename AdjState<F32, PER, T, CELLS>::R;
     R sc[3], w[3], c[8][3], B[3];
@@ -138,6 +159,10 @@ __device__ __forceinline__ void adj_step_back(const KParams& p, const AdjParams&
     for (int k = 0; k < 3; ++k) {
         v.bb[k] = Bb[k];
         v.w[k] = w[k];
+    }
+    if constexpr (F32) {
+        v.e = ex;                  // Bbar carries the state's scale at this step
+        adj_renorm(xb, ub, ex);
     }
     ((AdjRec<R>*)a.rec_val)[rec] = v;
 }
@@ -200,10 +225,20 @@ __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjP
     const double h = (p.q2m * 0.5) * dt;
     R* st = (R*)a.state;
     R xb[3], ub[3];
+    int ex = 0;                                                // float state: scale 2^ex
     if (seg == a.nseg - 1) {
+        double x0[3], m = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            xb[c] = (R)(2.0 / (double)N * (a.xfinal[3 * i + c] - a.targets[3 * i + c]));
+            x0[c] = 2.0 / (double)N * (a.xfinal[3 * i + c] - a.targets[3 * i + c]);
+            m = fmax(m, fabs(x0[c]));
+        }
+        if constexpr (F32) {
+            if (m > 0.0 && m <= 1.0e300) (void)frexp(m, &ex);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xb[c] = (R)(F32 ? ldexp(x0[c], -ex) : x0[c]);
             ub[c] = (R)0;
         }
     } else {
@@ -212,6 +247,7 @@ __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjP
             xb[c] = st[(size_t)c * N + i];
             ub[c] = st[(size_t)(3 + c) * N + i];
         }
+        if constexpr (F32) ex = a.state_exp[i];
     }
     constexpr int NC = F32 ? 9 : 6;
     R* buf = (R*)a.seg;
@@ -256,7 +292,7 @@ __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjP
                 u[d] = b[(size_t)(3 + d) * N];
             }
         }
-        adj_step_back<F32, PER, T, CELLS>(p, a, s, u, h, dt, xb, ub, true, (size_t)j * N + i);
+        adj_step_back<F32, PER, T, CELLS>(p, a, s, u, h, dt, xb, ub, ex, true, (size_t)j * N + i);
     }
     const size_t last = (size_t)a.K * N + i;
     if (seg == 0) {
@@ -264,11 +300,12 @@ __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjP
         S s0;
         s0.load(p, i);
         R v0[3] = {s0.v[0], s0.v[1], s0.v[2]};
-        adj_step_back<F32, PER, T, CELLS>(p, a, s0, v0, (p.q2m * 0.5) * (-0.5 * dt), 0.0, xb, ub, false, last);
+        adj_step_back<F32, PER, T, CELLS>(p, a, s0, v0, (p.q2m * 0.5) * (-0.5 * dt), 0.0, xb, ub, ex, false,
+                                          last);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            a.grad_xv0[6 * i + c] = (double)xb[c];
-            a.grad_xv0[6 * i + 3 + c] = (double)ub[c];
+            a.grad_xv0[6 * i + c] = ldexp((double)xb[c], ex);
+            a.grad_xv0[6 * i + 3 + c] = ldexp((double)ub[c], ex);
         }
     } else {
         adj_zero_record<R>(a, last);
@@ -277,6 +314,7 @@ __global__ void __launch_bounds__(256) k_adj_reverse(const KParams p, const AdjP
             st[(size_t)c * N + i] = xb[c];
             st[(size_t)(3 + c) * N + i] = ub[c];
         }
+        if constexpr (F32) a.state_exp[i] = ex;
     }
 }
 
@@ -290,6 +328,19 @@ __global__ void k_zero_f64(double* p, long long n)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// counts the entries of a[0..n) that are not finite into *count
+__global__ void __launch_bounds__(256) k_count_nonfinite(const double* __restrict__ a, long long n,
+                                                         unsigned long long* count)
+{
+    unsigned long long c = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        c += isfinite(a[i]) ? 0ull : 1ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
 
 template <bool F32, bool PER, typename T, bool CELLS>
@@ -332,12 +383,13 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t b_xv = al(sizeof(double) * 6 * N), b_tg = al(sizeof(double) * 3 * N),
                  b_ck = al(sizeof(double) * 9 * (size_t)nseg * N), b_sg = al(rs * ncomp * (size_t)K * N),
-                 b_st = al(rs * 6 * (size_t)N), b_key = al(sizeof(unsigned) * (size_t)M),
+                 b_st = al(rs * 6 * (size_t)N), b_se = al(sizeof(int) * (size_t)N), b_key = al(sizeof(unsigned) * (size_t)M),
                  b_val = al(sizeof(AdjRec<R>) * (size_t)M), b_tmp = al(std::max<size_t>(temp_bytes, 1)),
                  b_G = al(sizeof(double) * 24 * (size_t)ncell), b_out = al(sizeof(double) * 3 * nnodes),
                  b_gx = al(sizeof(double) * 6 * N), b_ls = al(sizeof(double)), b_xf = al(sizeof(double) * 3 * N),
-                 b_cnt = al(sizeof(unsigned) * (size_t)std::max<long long>(ncnt, 1));
-    const size_t total = b_xv + b_tg + b_ck + b_sg + b_st + 2 * b_key + 2 * b_val + b_tmp + b_G + b_out +
+                 b_cnt = al(sizeof(unsigned) * (size_t)std::max<long long>(ncnt, 1)),
+                 b_nf = al(sizeof(unsigned long long));
+    const size_t total = b_xv + b_tg + b_ck + b_sg + b_st + b_se + b_nf + 2 * b_key + 2 * b_val + b_tmp + b_G + b_out +
                          b_gx + b_ls + b_xf + 2 * b_cnt;
     char* buf = nullptr;
     HIP_TRY(ctx, hipMalloc((void**)&buf, total));
@@ -349,6 +401,8 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     a.ckpt = (double*)take(b_ck);
     a.seg = take(b_sg);
     a.state = take(b_st);
+    a.state_exp = (int*)take(b_se);
+    unsigned long long* d_nf = (unsigned long long*)take(b_nf);
     unsigned* keys0 = (unsigned*)take(b_key);
     unsigned* keys1 = (unsigned*)take(b_key);
     AdjRec<R>* vals0 = (AdjRec<R>*)take(b_val);
@@ -378,6 +432,7 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, G, ng);
         hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, a.loss, 1LL);
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, (double*)d_nf, 1LL);
         e = hipEventRecord(ctx->ev0, st);
     }
     if (e == hipSuccess) {
@@ -410,11 +465,23 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
+    // the outputs are scanned for entries that are not finite (the loss, the
+    // node gradient, the initial-state gradient)
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_count_nonfinite, dim3(1), dim3(64), 0, st, a.loss, 1LL, d_nf);
+        hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)std::min<size_t>((3 * nnodes + 255) / 256, 4096)),
+                           dim3(256), 0, st, d_out, (long long)(3 * nnodes), d_nf);
+        hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)std::min<long long>((6 * N + 255) / 256, 4096)),
+                           dim3(256), 0, st, a.grad_xv0, (long long)(6 * N), d_nf);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms_r, ctx->ev0, ctx->ev1);
+    unsigned long long nonfinite = 0;
     if (e == hipSuccess) e = hipMemcpy(dL_dB, d_out, sizeof(double) * 3 * nnodes, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(dL_dxv0, a.grad_xv0, sizeof(double) * 6 * N, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(loss, a.loss, sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&nonfinite, d_nf, sizeof nonfinite, hipMemcpyDeviceToHost);
     (void)hipFree(buf);
     if (e != hipSuccess) return fail(ctx, TPAMD_ERR_DEVICE, "adjoint failed: %s", hipGetErrorString(e));
     if (stats) {
@@ -426,6 +493,11 @@ int adjoint_run(tpamd_ctx* ctx, KParams p, const tpamd_field* B, const double* x
         stats->max_len = 1;
         stats->total_ms = ms_f + ms_r;
     }
+    if (nonfinite)
+        return fail(ctx, TPAMD_ERR_NONFINITE,
+                    "%llu entries of the loss and its gradients are not finite (the forward "
+                    "trajectory left the field, or a gradient exceeds the double range)",
+                    nonfinite);
     return TPAMD_OK;
 }
 

@@ -39,6 +39,22 @@ struct AdjRec {
     R w[3];
 };
 
+// Float records carry a power-of-two exponent: Bbar = bb * 2^e.  The exact
+// gradient of a chaotic flow grows like exp(lambda T) (|dL/dB| ~ 1e63 after
+// 2,048 steps of the 64^3 turbulent box), far past the float range, so the
+// float adjoint state is kept as a mantissa vector with one exponent per
+// particle, renormalised every step, and each record inherits its
+// particle's exponent.  The bin sums apply it in double.
+template <>
+struct AdjRec<float> {
+    float bb[3];
+    float w[3];
+    int e;
+};
+
+__device__ __forceinline__ double adj_rec_scale(const AdjRec<float>& v) { return ldexp(1.0, v.e); }
+__device__ __forceinline__ double adj_rec_scale(const AdjRec<double>&) { return 1.0; }
+
 // cells per bin: the bin's LDS tile holds 24 doubles per cell, 96 KiB.  The
 // tile is double for both precisions: on gfx950 an LDS float atomic add
 // (ds_add_f32) runs at ~1/7 the rate of the double one (ds_add_f64).
@@ -69,6 +85,7 @@ struct AdjParams {
     double* ckpt;             // [nseg][9][N]: frame position (3), wrap (3), velocity (3)
     void* seg;                // [K][9 or 6][N] recompute buffer (float or double)
     void* state;              // [6][N] adjoint state (xbar, ubar) between segment launches
+    int* state_exp;           // [N] its power-of-two exponent (float adjoint state only)
     unsigned* rec_key;        // [(K + 1) N] record keys: the cell index
     void* rec_val;            // [(K + 1) N] AdjRec<R>; rows without a step hold a zero record
     unsigned ncell;           // cells in the grid
@@ -214,14 +231,15 @@ __device__ __forceinline__ void adj_tile_add(double* tile, unsigned local, const
 {
     constexpr int kCells = AdjBin::kCells;
     double* t = tile + local;
+    const double sc = adj_rec_scale(v);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const R W = ((q & 1) ? v.w[0] : (R)1 - v.w[0]) *
                     (((q >> 1) & 1) ? v.w[1] : (R)1 - v.w[1]) *
                     ((q >> 2) ? v.w[2] : (R)1 - v.w[2]);
-        atomicAdd(t + (3 * q) * kCells, (double)(W * v.bb[0]));
-        atomicAdd(t + (3 * q + 1) * kCells, (double)(W * v.bb[1]));
-        atomicAdd(t + (3 * q + 2) * kCells, (double)(W * v.bb[2]));
+        atomicAdd(t + (3 * q) * kCells, (double)(W * v.bb[0]) * sc);
+        atomicAdd(t + (3 * q + 1) * kCells, (double)(W * v.bb[1]) * sc);
+        atomicAdd(t + (3 * q + 2) * kCells, (double)(W * v.bb[2]) * sc);
     }
 }
 

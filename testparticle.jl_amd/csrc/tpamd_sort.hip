@@ -132,7 +132,9 @@ __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins_seg(
     constexpr int RB = 4096;                  // owner sums: records grouped per batch
     constexpr int RPT = RB / T;
     static_assert(T == 2 * kCells, "two owner threads per cell");
-    static_assert(6 * RB * sizeof(float) == kCells * 24 * sizeof(double), "one LDS buffer for both");
+    // one LDS buffer for both: the tile, or 7 staged columns of RB records
+    constexpr int kLds = OWN ? 7 * RB * (int)sizeof(float) / (int)sizeof(double) : kCells * 24;
+    static_assert(!OWN || 7 * RB * sizeof(float) >= kCells * 24 * sizeof(double), "the tile fits");
     using Scan = hipcub::BlockScan<unsigned, T>;
     __shared__ typename Scan::TempStorage scan_tmp;
     __shared__ unsigned pre[kChunk];
@@ -140,10 +142,11 @@ __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins_seg(
     __shared__ unsigned cnt[OWN ? kCells : 1];
     __shared__ unsigned off[OWN ? kCells : 1];
     // the atomics' tile[(3 q + c) * kCells + cell], or the owners' staged
-    // records sb[v * RB + slot] (v: Bbar 0-2, weights 3-5)
-    __shared__ double lds[kCells * 24];
+    // records sb[v * RB + slot] (v: Bbar 0-2, weights 3-5, exponent 6)
+    __shared__ double lds[kLds];
     double* tile = lds;
     float* sb = reinterpret_cast<float*>(lds);
+    int* sbe = reinterpret_cast<int*>(lds) + 6 * RB;
     const int tid = threadIdx.x;
     const unsigned first = blockIdx.x << AdjBin::kShift;
     const int own = tid >> 1, half = tid & 1;
@@ -211,6 +214,7 @@ __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins_seg(
                         sb[3 * RB + d] = rv[q].w[0];
                         sb[4 * RB + d] = rv[q].w[1];
                         sb[5 * RB + d] = rv[q].w[2];
+                        if constexpr (OWN) sbe[d] = rv[q].e;
                     }
                 }
                 __syncthreads();
@@ -219,14 +223,15 @@ __global__ void __launch_bounds__(AdjBin::kThreads) k_adj_bins_seg(
                 for (unsigned i = i0; i < i1; ++i) {
                     const float b0 = sb[i], b1 = sb[RB + i], b2 = sb[2 * RB + i];
                     const float w0 = sb[3 * RB + i], w1 = sb[4 * RB + i], w2 = sb[5 * RB + i];
+                    const double sc = ldexp(1.0, sbe[i]);
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         const int q = 4 * half + k;
                         const float W = ((q & 1) ? w0 : 1.0f - w0) * (((q >> 1) & 1) ? w1 : 1.0f - w1) *
                                         ((q >> 2) ? w2 : 1.0f - w2);
-                        acc[3 * k] += (double)(W * b0);
-                        acc[3 * k + 1] += (double)(W * b1);
-                        acc[3 * k + 2] += (double)(W * b2);
+                        acc[3 * k] += (double)(W * b0) * sc;
+                        acc[3 * k + 1] += (double)(W * b1) * sc;
+                        acc[3 * k + 2] += (double)(W * b2) * sc;
                     }
                 }
                 __syncthreads();              // cnt / off / sb are read

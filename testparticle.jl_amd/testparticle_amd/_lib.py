@@ -16,7 +16,7 @@ ABI_VERSION = 1
 
 # tpamd_status
 OK, ERR_ARGUMENT, ERR_DT_TOO_SMALL, ERR_MAXITERS, ERR_ORDER = 0, 1, 2, 3, 4
-ERR_DEVICE, ERR_OOM, ERR_UNSUPPORTED, ERR_CAPACITY = 5, 6, 7, 8
+ERR_DEVICE, ERR_OOM, ERR_UNSUPPORTED, ERR_CAPACITY, ERR_NONFINITE = 5, 6, 7, 8, 9
 # enums
 FIELD_ZERO, FIELD_UNIFORM, FIELD_AFFINE, FIELD_DIPOLE, FIELD_GRID, FIELD_UXB = range(6)
 FIELD_TIME_GRID, FIELD_MESH = 6, 7
@@ -195,6 +195,8 @@ class Context:
             raise NotImplementedError(msg)
         if rc == ERR_CAPACITY:
             raise CapacityError(msg, int(stats.max_len) if stats is not None else -1)
+        if rc == ERR_NONFINITE:
+            raise FloatingPointError(msg)
         raise DeviceError(msg)
 
     def close(self):

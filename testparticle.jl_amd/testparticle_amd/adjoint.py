@@ -33,8 +33,13 @@ class AdjointResult:
     stats: object
 
 
-def loss_gradient(prob, *, dt, targets, precision: str = "fp32c", ckpt_interval: int = 64,
+def loss_gradient(prob, *, dt, targets, precision: str = "fp64", ckpt_interval: int = 64,
                   device: int = 0) -> AdjointResult:
+    """One forward pass and one reverse sweep.  precision "fp64" differentiates
+    the FP64 solve (what ForwardDiff through the reference computes);
+    "fp32c" runs the forward, recompute and reverse steps in float over float
+    storage, with the adjoint state scaled by a per-particle power of two.
+    Raises FloatingPointError when the loss or a gradient entry is not finite."""
     q2m, _m, E, B = prob.p[:4]
     if not isinstance(E, ZeroField):
         raise NotImplementedError("the adjoint covers E = 0")
@@ -59,7 +64,7 @@ def loss_gradient(prob, *, dt, targets, precision: str = "fp32c", ckpt_interval:
                          st.kernel_ms, st.aux_ms, st)
 
 
-def trajectory_loss(B_values, xv0, *, axes, bc, q2m, tspan, dt, targets, precision="fp32c",
+def trajectory_loss(B_values, xv0, *, axes, bc, q2m, tspan, dt, targets, precision="fp64",
                     ckpt_interval: int = 64):
     """torch autograd entry: loss(B_values (3, nx, ny, nz), xv0 (N, 6)) with
     gradients to both tensors from the adjoint sweep."""
