@@ -1,17 +1,21 @@
 # TestParticleAMD.jl -- thin ccall layer from TestParticle.jl to libtpamd.so.
 #
-# Loaded as a package extension of TestParticle.jl, it adds a method of
-# `solve` that dispatches on the device backend `MI355X()`, mirroring the
-# KernelAbstractions entry point
+# A package next to TestParticle.jl (Project.toml beside src/): it adds a
+# method of `solve` that dispatches on the device backend `MI355X()`,
+# mirroring the KernelAbstractions entry point
 #     solve(prob, ::Boris{false}, backend::KA.Backend, ensemblealg; dt, ...)
 # (src/boris/boris_kernel.jl:312-382), and extends it to MultistepBoris and the
 # adaptive solvers; and methods that route the reference's own
 # `solve(prob, Boris(); dt, ...)` to the GPU when p carries device fields.
 # Fields are described with the device menu (DeviceField) rather than Julia
-# closures, since a GPU cannot call host code.
+# closures, since a GPU cannot call host code.  `trajectory_loss` and
+# `loss_gradient` wrap the adjoint; the ChainRulesCore extension
+# (ext/TestParticleAMDChainRulesCoreExt.jl) gives `trajectory_loss` its rrule.
 #
-# Untested here: the build container has no Julia.  The C ABI it calls is
-# include/tpamd.h, which the Python host tests exercise.
+# The build container has no Julia, so this file is not run there; the
+# struct mirrors and the ccall signatures below are checked against
+# include/tpamd.h by the Python tests (tests/test_julia_binding.py), and the
+# C ABI itself is exercised through ctypes.
 module TestParticleAMD
 
 using TestParticle
@@ -19,7 +23,7 @@ using TestParticle: TraceProblem, Boris, MultistepBoris, build_solution
 using SciMLBase: EnsembleSolution, ReturnCode, LinearInterpolation
 using StaticArrays
 
-const libtpamd = joinpath(@__DIR__, "..", "_build", "libtpamd.so")
+const libtpamd = joinpath(@__DIR__, "..", "..", "_build", "libtpamd.so")
 
 struct MI355X
     device::Int
@@ -32,7 +36,7 @@ Base.@kwdef mutable struct SolveDesc
     adaptive::Int32 = 0
     order_N::Int32 = 2
     nsub::Int32 = 1
-    form::Int32 = 1
+    form::Int32 = 0           # 0 reference (atan, sincos: multistep_boris.jl:94-120), 1 rational
     precision::Int32 = 0
     save_start::Int32 = 1
     save_end::Int32 = 1
@@ -62,6 +66,7 @@ Base.@kwdef mutable struct SolveDesc
     reserved::Int32 = 0
 end
 
+# mirrors tpamd_stats (include/tpamd.h)
 mutable struct Stats
     kernel_ms::Float64
     total_ms::Float64
@@ -69,8 +74,9 @@ mutable struct Stats
     nt::Int64
     nout::Int64
     max_len::Int64
+    aux_ms::Float64
 end
-Stats() = Stats(0, 0, 0, 0, 0, 0)
+Stats() = Stats(0, 0, 0, 0, 0, 0, 0)
 
 const _ctx = Ref{Ptr{Cvoid}}(C_NULL)
 
@@ -84,12 +90,21 @@ function context(dev::Int)
     return _ctx[]
 end
 
+# tpamd_status: 1-4 are the reference's ArgumentErrors (boris.jl:349-356,
+# types.jl:70-75); 9 (TPAMD_ERR_NONFINITE) a loss or gradient that is not finite
 function check(ctx, rc)
     rc == 0 && return
     msg = unsafe_string(ccall((:tpamd_last_error, libtpamd), Cstring, (Ptr{Cvoid},), ctx))
     rc in (1, 2, 3, 4) && throw(ArgumentError(msg))
+    rc == 9 && throw(DomainError(rc, msg))
     error(msg)
 end
+
+# fixed step or adaptive, from the algorithm's type (src/types.jl:51-107): a
+# fixed-step MultistepBoris{N, false}(; n) still carries safety = 0.1
+is_adaptive(::Boris{A}) where {A} = A
+is_adaptive(::MultistepBoris{N, A}) where {N, A} = A
+multistep_order(::MultistepBoris{N}) where {N} = N
 
 # --- device field menu -------------------------------------------------------
 abstract type DeviceField end
@@ -216,7 +231,7 @@ end
     solve(prob::TraceProblem, alg, backend::MI355X, [ensemblealg]; dt, trajectories=1,
           savestepinterval=1, isoutside=nothing, save_start=true, save_end=true,
           save_everystep=true, save_fields=false, save_work=false, maxiters=1_000_000,
-          precision=:fp64)
+          precision=:fp64, form=:reference)
 
 Device counterpart of `solve(prob, Boris(), backend; ...)`
 (src/boris/boris_kernel.jl:312-382) for Boris, MultistepBoris and the adaptive
@@ -226,7 +241,8 @@ function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensembleal
         dt = 0.0, trajectories::Int = 1, savestepinterval::Int = 1, isoutside = nothing,
         save_start::Bool = true, save_end::Bool = true, save_everystep::Bool = true,
         save_fields::Bool = false, save_work::Bool = false,
-        maxiters::Int = 1_000_000, precision::Symbol = :fp64, nout_cap::Int = 0)
+        maxiters::Int = 1_000_000, precision::Symbol = :fp64, form::Symbol = :reference,
+        nout_cap::Int = 0)
     ctx = context(backend.device)
     q2m, m, E, B, _ = prob.p
     T = Float64
@@ -243,10 +259,11 @@ function TestParticle.solve(prob::TraceProblem, alg, backend::MI355X, ensembleal
     # record width, as _prepare_saved_data builds it (boris.jl:405-440)
     W = 6 + (save_fields ? 6 : 0) + (save_work ? 4 : 0)
     if alg isa MultistepBoris
-        d.alg = 1; d.order_N = typeof(alg).parameters[1]; d.nsub = alg.n
+        d.alg = 1; d.order_N = multistep_order(alg); d.nsub = alg.n
+        d.form = form === :rational ? 1 : 0
     end
-    d.adaptive = alg.safety > 0 ? 1 : 0
-    d.safety = alg.safety
+    d.adaptive = is_adaptive(alg) ? 1 : 0
+    d.safety = is_adaptive(alg) ? alg.safety : 0.0
     isoutside === nothing || apply_boundary!(d, isoutside)
     nt, nout = Ref{Int64}(0), Ref{Int64}(0)
     if d.adaptive == 0
@@ -327,6 +344,66 @@ function prepare(E::DeviceField, B::DeviceField; device::MI355X = MI355X(),
     mm = m === nothing ? species.m : m
     return (qq / mm, mm, E, B, TestParticle.ZeroField())
 end
+
+# --- gradients -----------------------------------------------------------------
+"""
+    loss_gradient(A, xv0; x, y, z, bc = 2, q2m, tspan, dt, targets, precision = :fp64,
+                  ckpt_interval = 64, device = 0) -> (loss, dL_dA, dL_dxv0)
+
+Loss L = mean_i |x_i(T) - x*_i|^2 of the final positions of a fixed-step
+standard Boris solve with E = 0 and B the gridded field A (3 x nx x ny x nz
+on the uniform ranges x, y, z; `bc` 0 fill NaN, 1 clamp, 2 periodic; `cells`
+adds the cell-replicated device copy), and its gradients with respect to every value
+of A and every initial state (xv0 and targets are 6 x N and 3 x N).  One
+forward pass with FP64 checkpoints every `ckpt_interval` steps, then a reverse
+sweep (tpamd_adjoint).  The reference differentiates the same solve with
+ForwardDiff (test/test_derivatives.jl).  `precision = :fp32c` runs the steps
+in float over float storage.  A loss or gradient that is not finite throws
+a DomainError.
+"""
+function loss_gradient(A::Array{T, 4}, xv0::AbstractMatrix, targets::AbstractMatrix;
+        x::AbstractRange, y::AbstractRange, z::AbstractRange, bc::Integer = 2, q2m::Real,
+        tspan, dt::Real, precision::Symbol = :fp64, ckpt_interval::Integer = 64,
+        cells::Bool = true, device::Integer = 0) where {T <: Union{Float32, Float64}}
+    size(A, 1) == 3 || throw(ArgumentError("A must be 3 x nx x ny x nz"))
+    size(xv0, 1) == 6 && size(targets) == (3, size(xv0, 2)) ||
+        throw(ArgumentError("xv0 must be 6 x N and targets 3 x N"))
+    ctx = context(device)
+    storage = precision === :fp32c ? Int32(1) : Int32(0)
+    hB = handle(ctx, GridDeviceField(A, x, y, z, Int32(bc), storage), IdDict())
+    # the cell-replicated copy: one cache line per gather (same values)
+    cells && check(ctx, ccall((:tpamd_field_grid_cells, libtpamd), Int32,
+                              (Ptr{Cvoid}, Ptr{Cvoid}, Int32), ctx, hB, Int32(1)))
+    N = size(xv0, 2)
+    d = SolveDesc(q2m = q2m, t0 = tspan[1], t1 = tspan[2], dt = dt, save_start = 0,
+                  save_everystep = 0, precision = precision === :fp32c ? 1 : 0, nout_cap = 1)
+    x0 = Matrix{Float64}(xv0)
+    tg = Matrix{Float64}(targets)
+    loss = Ref(0.0)
+    gA = Array{Float64}(undef, size(A))           # node layout [k][j][i][c] = 3 x nx x ny x nz
+    gx = Matrix{Float64}(undef, 6, N)             # [N][6] = 6 x N
+    st = Ref(Stats())
+    try
+        GC.@preserve x0 tg gA gx begin
+            rc = ccall((:tpamd_adjoint, libtpamd), Int32,
+                (Ptr{Cvoid}, Ref{SolveDesc}, Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Int32,
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{Stats}),
+                ctx, d, hB, x0, N, tg, Int32(ckpt_interval), loss, gA, gx, st)
+            check(ctx, rc)
+        end
+    finally
+        ccall((:tpamd_field_free, libtpamd), Int32, (Ptr{Cvoid}, Ptr{Cvoid}), ctx, hB)
+    end
+    return loss[], gA, gx
+end
+
+"""
+    trajectory_loss(A, xv0, targets; kw...) -> loss
+
+The loss of `loss_gradient` alone.  With ChainRulesCore loaded, its rrule
+returns the adjoint's gradients for A and xv0 from the same call.
+"""
+trajectory_loss(A, xv0, targets; kw...) = loss_gradient(A, xv0, targets; kw...)[1]
 
 function apply_boundary!(d::SolveDesc, spec)
     for s in (spec isa Tuple ? spec : (spec,))
