@@ -7,12 +7,17 @@ with u = (4e5, 0, 0) m/s derived in-kernel, 4,194,304 protons (1-100 keV,
 isotropic), MultistepBoris4(n=2), dt = T_g(B0)/40, 5,000 steps, start and end
 records saved.  One "step" of this script = one full solve (all particles,
 all 5,000 Boris steps) with initial states and records resident in HBM.
-`value` is the FP32-compensated pusher's particle-steps/s; the FP64 pusher is
-timed the same way beside it, and the FP32C final positions are compared with
+`value` is the FP64 pusher's particle-steps/s in the reference's own
+multistep arithmetic (atan, sincos), bit-identical to the oracle; the
+FP32-compensated pusher and the rational-coefficient form are timed the same
+way beside it ("variants"), and the FP32C final positions are compared with
 FP64 at several horizons next to the FP64 chaos floor (FP64 against FP64
 started one ulp away).
 
 Secondary sections (same JSON line, key "secondary"):
+  gyration 1,024 protons in a uniform B, standard Boris, 6,400 steps, a record
+           every 10 (the reference's CPU-runnable workload), FP64 and FP32C,
+           with the oracle over the whole ensemble beside them
   dipole   analytic Earth dipole, 1,048,576 protons, standard Boris, per-particle
            dt = T_g/32, 10,000 steps, removal inside 1 R_E (FP64 and FP32C)
   batch    64 uneven turbulent-box instances (32^3..128^3, ~2e7 protons),
@@ -51,7 +56,7 @@ STATE_BYTES, LEN_BYTES, RECORD_BYTES = 6 * 8, 2 * 4, 7 * 8
 # with E = 0 (37) + position update (6) + dipole evaluation (27)
 DIPOLE_FLOP = 37 + 6 + 27
 QI, MI = 1.602176634e-19, 1.67262192369e-27   # proton charge and mass (constants.jl:4-20)
-SECTIONS = ("headline", "dipole", "batch", "adjoint")
+SECTIONS = ("headline", "gyration", "dipole", "batch", "adjoint")
 
 
 def log(*a):
@@ -90,10 +95,67 @@ def rel_pos_err(xa, xb):
             "max": float(np.max(d))}
 
 
+def host_info():
+    """The host cores the CPU baselines run on: the threads used (the box
+    grants one GPU's process 16 host cores, so the oracle's OpenMP team is
+    sized to min(16, affinity)), the affinity count, the CPU model and the SMT
+    state."""
+    info = {"threads": cpu_threads()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity_cpus"] = os.cpu_count()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            info["smt_active"] = f.read().strip() == "1"
+    except OSError:
+        info["smt_active"] = None
+    return info
+
+
+def oracle_calibration():
+    """One thread on the reference's serial benchmark problem
+    (benchmark/bench_serial.jl:10-45: proton, uniform B = (0, 0, 1e-8) T,
+    E = 0, v0 = (1e5, 0, 0), dt = 1 ns, 1e8 steps, a record every 1e5
+    steps), published at 1.08e8 particle-steps/s per core
+    (docs/src/index.md:48-54)."""
+    import oracle as O
+
+    cal = O.solve(np.array([[0.0, 0.0, 0.0, 1.0e5, 0.0, 0.0]]), O.zero(), O.uniform([0.0, 0.0, 1.0e-8]),
+                  q2m=QI / MI, tspan=(0.0, 0.1), dt=1.0e-9, savestepinterval=100000,
+                  maxiters=200_000_000, nthreads=1)
+    return {"problem": "bench_serial.jl: 1 proton, uniform B, 1e8 Boris steps", "threads": 1,
+            "value": 1.0e8 / cal.elapsed, "published_per_core": 1.08e8}
+
+
+def oracle_rate(run, n_total, nthreads, target_s, what):
+    """Times run(n) (the oracle over a sample of n particles; returns its
+    elapsed seconds and the Boris steps it advanced) on a probe, then on a
+    sample sized to ~target_s of CPU work, and returns the cpu_baseline
+    record."""
+    probe = min(max(nthreads * 4, 64), n_total)
+    el, steps = run(probe)
+    per_particle_s = max(el, 1e-6) / probe
+    n = int(min(n_total, max(probe, target_s / per_particle_s)))
+    el, steps = run(n)
+    return {"value": steps / el, "unit": "particle-steps/s", "cores": nthreads,
+            "kind": "port", "host": host_info(),
+            "sample": f"{n} of {n_total} particles, {steps} particle-steps, {what}, {el:.1f} s wall"}
+
+
 def cpu_baseline(w, A64, ax, nthreads, target_s=12.0):
     """The oracle (CPU restatement of the reference's EnsembleThreads solve:
-    FP64, reference-form MultistepBoris4 with atan/sincos) on all host cores,
-    over a bounded sample of the same particles."""
+    FP64, reference-form MultistepBoris4 with the FDLIBM atan and sin/cos) on
+    the host cores, over a bounded sample of the same particles: the same
+    arithmetic as the headline value."""
     import oracle as O
 
     B = O.grid(np.ascontiguousarray(A64.transpose(3, 2, 1, 0)), [a[0] for a in ax],
@@ -101,30 +163,33 @@ def cpu_baseline(w, A64, ax, nthreads, target_s=12.0):
     E = O.uxb(w.extra["u"], B)
     kw = dict(q2m=w.q2m, tspan=w.tspan, dt=w.dt, alg=O.ALG_MULTISTEP, order_N=4, nsub=2,
               form=O.FORM_REFERENCE, save_everystep=False, nthreads=nthreads)
-    probe = max(nthreads * 4, 64)
-    r = O.solve(w.xv0[:probe], E, B, **kw)
-    rate = probe * w.nsteps / max(r.elapsed, 1e-6)
-    n = int(min(len(w.xv0), max(probe, rate * target_s / w.nsteps)))
-    r = O.solve(w.xv0[:n], E, B, **kw)
-    # calibration: one thread on the reference's serial benchmark problem
-    # (benchmark/bench_serial.jl:10-45: proton, uniform B = (0, 0, 1e-8) T,
-    # E = 0, v0 = (1e5, 0, 0), dt = 1 ns, 1e8 steps, a record every 1e5 steps),
-    # published at 1.08e8 particle-steps/s per core (docs/src/index.md:48-54)
-    cal = O.solve(np.array([[0.0, 0.0, 0.0, 1.0e5, 0.0, 0.0]]), O.zero(), O.uniform([0.0, 0.0, 1.0e-8]),
-                  q2m=QI / MI, tspan=(0.0, 0.1), dt=1.0e-9, savestepinterval=100000,
-                  maxiters=200_000_000, nthreads=1)
-    return {"value": n * w.nsteps / r.elapsed, "unit": "particle-steps/s", "cores": nthreads,
-            "kind": "port",
-            "sample": f"{n} of {len(w.xv0)} particles x {w.nsteps} steps, FP64 "
-                      f"MultistepBoris4(n=2) reference form, {r.elapsed:.1f} s wall",
-            "calibration": {"problem": "bench_serial.jl: 1 proton, uniform B, 1e8 Boris steps",
-                            "threads": 1, "value": 1.0e8 / cal.elapsed,
-                            "published_per_core": 1.08e8}}
+    def run(n):
+        r = O.solve(w.xv0[:n], E, B, **kw)
+        return r.elapsed, int(r.steps.sum())
+
+    out = oracle_rate(run, len(w.xv0), nthreads, target_s,
+                      f"FP64 MultistepBoris4(n=2) reference form, {w.nsteps} steps each")
+    out["calibration"] = oracle_calibration()
+    return out
 
 
 # ---------------------------------------------------------------------------
 # headline: gridded turbulent box
 # ---------------------------------------------------------------------------
+# (precision, multistep form) variants; the first is `value`: FP64 in the
+# reference's own arithmetic (atan, sincos), bit-identical to the oracle
+HEADLINE_VARIANTS = (("fp64", "reference"), ("fp32c", "reference"), ("fp64", "rational"),
+                     ("fp32c", "rational"))
+
+
+def _kernel_name(prec, form, layout):
+    ns = "n=2" if form == "rational" else "reference form n=2"
+    lay = "cells" if layout == "cells" else "nodes"
+    if prec == "fp64":
+        return f"k_fixed<S64G<UxB, {ns}, periodic, double, {lay}>>"
+    return f"k_fixed<S32G<UxB, {ns}, periodic, {lay}>>"
+
+
 def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
     N, n, nsteps = args.particles, args.grid, args.nsteps
     log(f"[bench] generating {n}^3 turbulent field and {N} protons")
@@ -135,45 +200,63 @@ def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
     x0p = w.xv0.copy()
     x0p[:, :3] = np.nextafter(x0p[:, :3], np.inf)        # one ulp away, for the chaos floor
     res = {}
-    for prec, storage in (("fp32c", "f32"), ("fp64", "f64")):
+    for prec, storage in (("fp64", "f64"), ("fp32c", "f32")):
         A = A64 if storage == "f64" else A64.astype(np.float32)
         B = TP.GridField(A, *ax, bc=TP.PeriodicBC(), layout=args.layout)
         E = TP.UxBField(w.extra["u"], B)
         param = (w.q2m, TP.mi, E, B, TP.ZeroField())
         prob = TP.TraceProblem(w.xv0, w.tspan, param)
-        log(f"[bench] {prec}: uploading and warming up")
-        ens = DeviceEnsemble(prob, alg, w.xv0, dt=w.dt, save_everystep=False, precision=prec)
-        wall, kms, st = timed_runs(ens.run, torch, args.warmup, args.steps)
-        assert st.particle_steps == N * nsteps, st.particle_steps
-        r = dict(wall=wall, kernel_ms=kms, storage=storage, final={nsteps: ens.final_states()[:, :3]})
-        log(f"[bench] {prec}: {wall * 1e3:.1f} ms/solve, kernel {kms:.1f} ms, "
-            f"{N * nsteps / wall:.3e} particle-steps/s")
-        del ens
-        if prec == "fp32c" and args.accuracy:
-            # the host-buffer entry point: PCIe copies in and out included
-            t0 = time.perf_counter()
-            TP.solve(prob, alg, dt=w.dt, save_everystep=False, precision=prec)
-            r["host_wall"] = time.perf_counter() - t0
-        if args.accuracy:
-            for h in horizons:
-                starts = [("x0", w.xv0)] + ([("x0p", x0p)] if prec == "fp64" else [])
-                for tag, xv in starts:
-                    if h == nsteps and tag == "x0":
-                        continue
-                    pr = TP.TraceProblem(xv, (0.0, h * w.dt), param)
-                    e = DeviceEnsemble(pr, alg, xv, dt=w.dt, save_everystep=False, precision=prec)
-                    e.run()
-                    key = "final" if tag == "x0" else "final_p"
-                    r.setdefault(key, {})[h] = e.final_states()[:, :3]
-                    del e
-        res[prec] = r
+        for form in ("reference", "rational"):
+            key = (prec, form)
+            if args.variants and f"{prec}_{form}" not in args.variants and key != HEADLINE_VARIANTS[0]:
+                continue
+            first = key == HEADLINE_VARIANTS[0]
+            log(f"[bench] {prec} {form}: uploading and warming up")
+            ens = DeviceEnsemble(prob, alg, w.xv0, dt=w.dt, save_everystep=False, precision=prec, form=form)
+            # the value's variant takes the full count; the others at most 5 timed solves
+            steps = args.steps if first else min(args.steps, 5)
+            wall, kms, st = timed_runs(ens.run, torch, args.warmup if first else 1, steps)
+            assert st.particle_steps == N * nsteps, st.particle_steps
+            r = dict(wall=wall, kernel_ms=kms, storage=storage, steps=steps,
+                     final={nsteps: ens.final_states()[:, :3]})
+            log(f"[bench] {prec} {form}: {wall * 1e3:.1f} ms/solve, kernel {kms:.1f} ms, "
+                f"{N * nsteps / wall:.3e} particle-steps/s")
+            del ens
+            if first and args.accuracy:
+                # the host-buffer entry point: PCIe copies in and out included
+                t0 = time.perf_counter()
+                TP.solve(prob, alg, dt=w.dt, save_everystep=False, precision=prec, form=form)
+                r["host_wall"] = time.perf_counter() - t0
+            if args.accuracy and form == "reference":
+                for h in horizons:
+                    starts = [("x0", w.xv0)] + ([("x0p", x0p)] if prec == "fp64" else [])
+                    for tag, xv in starts:
+                        if h == nsteps and tag == "x0":
+                            continue
+                        pr = TP.TraceProblem(xv, (0.0, h * w.dt), param)
+                        e = DeviceEnsemble(pr, alg, xv, dt=w.dt, save_everystep=False, precision=prec,
+                                           form=form)
+                        e.run()
+                        r.setdefault("final" if tag == "x0" else "final_p", {})[h] = e.final_states()[:, :3]
+                        del e
+            res[key] = r
         del B, E, prob
         torch.cuda.empty_cache()
     ps = N * nsteps
-    r32 = res["fp32c"]
-    value = ps / r32["wall"]
-    bytes_launch = ps * GATHER_BYTES["f32"] + N * (48 + 2 * 56)
-    achieved = bytes_launch / (r32["kernel_ms"] * 1e-3)
+    vkey = HEADLINE_VARIANTS[0]
+    rv = res[vkey]
+    value = ps / rv["wall"]
+    # algorithmic bytes of one launch: the gather per particle-step, the state
+    # read and the two records written per particle
+    bytes_launch = ps * GATHER_BYTES["f64"] + N * (STATE_BYTES + LEN_BYTES + 2 * RECORD_BYTES)
+    achieved = bytes_launch / (rv["kernel_ms"] * 1e-3)
+    variants = {}
+    for key, r in res.items():
+        variants[f"{key[0]}_{key[1]}"] = {
+            "value": ps / r["wall"], "ms_per_solve": r["wall"] * 1e3, "kernel_ms": r["kernel_ms"],
+            "timed_solves": r["steps"], "storage": r["storage"], "kernel": _kernel_name(*key, args.layout),
+            "algorithmic_bytes_per_particle_step": GATHER_BYTES[r["storage"]],
+            "hbm_frac": ps * GATHER_BYTES[r["storage"]] / (r["kernel_ms"] * 1e-3) / HBM_PEAK}
     out = {
         "metric": "particle-steps/s (gridded Boris) FP32-comp vs FP64 at <=1e-5 err; adjoint cost",
         "value": value,
@@ -181,40 +264,50 @@ def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r32["wall"] * 1e3,
+        "ms_per_step": rv["wall"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic (seeded k^-5/3 turbulent field and particle states)",
         "config": {"workload": f"turbulent_box_{n}^3_multistep4_n2", "particles": N,
                    "grid": n, "steps_per_solve": nsteps, "alg": "MultistepBoris4(n=2)",
+                   "multistep_form": "reference (atan, sincos, Taylor branch: multistep_boris.jl:94-120)",
                    "E": "-u x B in-kernel", "saves": "start+end", "field_layout": args.layout,
+                   "field_storage": "f64",
                    "parallelism": "single GPU, one thread per particle"},
-        "fp64_value": ps / res["fp64"]["wall"],
-        "fp32c_over_fp64": res["fp64"]["wall"] / r32["wall"],
-        "kernel_ms": {"fp32c": r32["kernel_ms"], "fp64": res["fp64"]["kernel_ms"]},
+        "value_note": "FP64 in the reference's arithmetic, bit-identical to the oracle (the <=1e-5 "
+                      "qualifier holds for FP64 only: the flow is chaotic, see fp32c_error)",
+        "variants": variants,
     }
-    if "host_wall" in r32:
-        out["fp32c_host_buffers"] = {"value": ps / r32["host_wall"], "ms_per_solve": r32["host_wall"] * 1e3,
-                                     "note": "tpamd_solve from host arrays: PCIe copies and host "
-                                             "output allocation included"}
-    if args.accuracy:
+    if ("fp32c", "reference") in res:
+        out["fp32c_over_fp64"] = res[("fp64", "reference")]["wall"] / res[("fp32c", "reference")]["wall"]
+    if "host_wall" in rv:
+        out["host_buffers"] = {"value": ps / rv["host_wall"], "ms_per_solve": rv["host_wall"] * 1e3,
+                               "note": "tpamd_solve from host arrays (FP64, reference form): PCIe "
+                                       "copies and host output allocation included"}
+    fin = {k: r["final"][nsteps] for k, r in res.items()}
+    if ("fp64", "rational") in fin:
+        out["rational_vs_reference_fp64"] = rel_pos_err(fin[("fp64", "rational")], fin[("fp64", "reference")])
+    if args.accuracy and ("fp32c", "reference") in res:
         acc = {}
-        x64 = res["fp64"]["final"]
+        x64 = res[("fp64", "reference")]
+        x32 = res[("fp32c", "reference")]
         for h in horizons:
-            acc[str(h)] = {"fp32c_vs_fp64": rel_pos_err(r32["final"][h], x64[h]),
-                           "fp64_one_ulp_floor": rel_pos_err(res["fp64"]["final_p"][h], x64[h])}
+            acc[str(h)] = {"fp32c_vs_fp64": rel_pos_err(x32["final"][h], x64["final"][h]),
+                           "fp64_one_ulp_floor": rel_pos_err(x64["final_p"][h], x64["final"][h])}
+        le = [h for h in horizons if acc[str(h)]["fp32c_vs_fp64"]["median"] <= 1e-5]
         out["fp32c_error"] = {
-            "metric": "|x_fp32c - x_fp64| / |x_fp64| of the final positions, per horizon (steps)",
+            "metric": "|x_fp32c - x_fp64| / |x_fp64| of the final positions, per horizon (steps), "
+                      "reference form",
             "chaos_floor": "fp64_one_ulp_floor = FP64 started one ulp away in position vs FP64",
+            "longest_horizon_with_median_le_1e-5": max(le) if le else None,
             "horizons": acc}
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                        "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                       "kernel": "k_fixed<S32G<UxB, n=2, periodic, cells>>" if args.layout == "cells"
-                       else "k_fixed<S32G<UxB, n=2, periodic, nodes>>",
-                       "kernel_ms": r32["kernel_ms"],
-                       "algorithmic_bytes_per_particle_step": GATHER_BYTES["f32"],
+                       "kernel": _kernel_name(*vkey, args.layout),
+                       "kernel_ms": rv["kernel_ms"],
+                       "algorithmic_bytes_per_particle_step": GATHER_BYTES["f64"],
                        "algorithmic_bytes_per_launch": bytes_launch}
     # HBM traffic per launch: FETCH_SIZE + WRITE_SIZE per particle-step from the
     # rocprofv3 --pmc passes recorded under profiles/ (scaled by the copy-probe
@@ -223,15 +316,67 @@ def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
     if os.path.exists(prof):
         with open(prof) as f:
             t = json.load(f)
-        if t.get("layout") == args.layout:
-            out["roofline"]["traffic"] = ps * (t["fetch_bytes_per_particle_step"] +
-                                               t["write_bytes_per_particle_step"])
+        v = t.get("variants", {}).get(f"{vkey[0]}_{vkey[1]}")
+        if t.get("layout") == args.layout and v:
+            out["roofline"]["traffic"] = ps * (v["fetch_bytes_per_particle_step"] +
+                                               v["write_bytes_per_particle_step"])
             out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
     if args.cpu_baseline:
-        log("[bench] CPU baseline (oracle, all host cores)")
+        log("[bench] CPU baseline (oracle, host cores)")
         out["cpu_baseline"] = cpu_baseline(w, A64, ax, cpu_threads())
     else:
         out["cpu_baseline"] = None
+    return out
+
+
+# ---------------------------------------------------------------------------
+# secondary: plain Boris in a uniform field (the CPU-runnable workload)
+# ---------------------------------------------------------------------------
+def bench_gyration(args, torch, TP, W, DeviceEnsemble):
+    """1,024 protons, uniform B = (0, 0, 1e-8) T, E = 0, positions in
+    [-1, 1]^3 m, Maxwellian velocities (thermal speed 1e5 m/s), dt =
+    gyroperiod/64, 6,400 steps, a record every 10 steps: FP64 and FP32C on the
+    device, the oracle over the whole ensemble on the host cores, FP64
+    bit-identity and |v|^2 drift."""
+    import oracle as O
+
+    w = W.uniform_gyration(N=1024)
+    N = len(w.xv0)
+    prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, TP.ZeroField(), TP.UniformField(w.extra["B"]),
+                                            TP.ZeroField()))
+    out = {"workload": "uniform_gyration_1024", "particles": N, "steps": w.nsteps,
+           "savestepinterval": w.savestepinterval, "alg": "Boris"}
+    res = {}
+    for prec in ("fp64", "fp32c"):
+        ens = DeviceEnsemble(prob, TP.Boris(), w.xv0, dt=w.dt, savestepinterval=w.savestepinterval,
+                             precision=prec)
+        wall, kms, st = timed_runs(ens.run, torch, 2, max(args.aux_steps, 5))
+        u, t, ln, _r = ens.host()
+        res[prec] = (u, t, ln)
+        out[prec] = {"value": st.particle_steps / wall, "unit": "particle-steps/s", "ms_per_solve": wall * 1e3,
+                     "kernel_ms": kms, "records_per_particle": int(ln[0]),
+                     "note": "6.6e6 particle-steps: launch and record copy dominate at this size"}
+        del ens
+    o = O.solve(w.xv0, O.zero(), O.uniform(w.extra["B"]), q2m=w.q2m, tspan=w.tspan, dt=w.dt,
+                savestepinterval=w.savestepinterval, nthreads=cpu_threads())
+    u64, t64, ln64 = res["fp64"]
+    out["fp64_bit_identical_to_oracle"] = bool(np.array_equal(ln64, o.len) and
+                                               np.array_equal(u64[:, :o.u.shape[1]], o.u) and
+                                               np.array_equal(t64[:, :o.t.shape[1]], o.t))
+    v0 = np.sum(w.xv0[:, 3:] ** 2, axis=1)
+    for prec, (u, _t, ln) in res.items():
+        vend = u[np.arange(N), ln - 1, 3:]
+        d = np.abs(np.sum(vend ** 2, axis=1) - v0) / v0
+        out[prec]["energy_drift"] = {"median": float(np.median(d)), "max": float(np.max(d))}
+    out["fp32c_vs_fp64"] = rel_pos_err(res["fp32c"][0][np.arange(N), res["fp32c"][2] - 1, :3],
+                                       u64[np.arange(N), ln64 - 1, :3])
+    steps = int(o.steps.sum())
+    out["cpu_baseline"] = {"value": steps / o.elapsed, "unit": "particle-steps/s", "cores": cpu_threads(),
+                           "kind": "port", "host": host_info(),
+                           "sample": f"the whole ensemble: {N} particles x {w.nsteps} steps, FP64 Boris, "
+                                     f"{o.elapsed:.3f} s wall"}
+    log(f"[bench] gyration: fp64 {out['fp64']['value']:.3e}, fp32c {out['fp32c']['value']:.3e}, "
+        f"cpu {out['cpu_baseline']['value']:.3e} particle-steps/s")
     return out
 
 
@@ -253,13 +398,26 @@ def bench_dipole(args, torch, TP, W, DeviceEnsemble):
                              save_everystep=False, isoutside=TP.InsideSphere(TP.Re),
                              precision=prec)
         wall, kms, st = timed_runs(ens.run, torch, 1, args.aux_steps)
-        u, _t, ln, ret = ens.host()
-        fin[prec] = (u[np.arange(len(ln)), np.maximum(ln - 1, 0)], ret)
+        u, tt, ln, ret = ens.host()
+        last = np.maximum(ln - 1, 0)
+        ar = np.arange(len(ln))
+        # With save_everystep = false a removed particle still gets its closing
+        # record and returns Success (boris.jl:523-551): it is removed when that
+        # record's time is before its t1 (a survivor's closing record is at t1)
+        steps_done = np.rint(tt[ar, last] / w.dts).astype(np.int64)
+        removed = steps_done < w.nsteps
+        fin[prec] = (u[ar, last], removed)
         rate = st.particle_steps / wall
         flops = st.particle_steps * DIPOLE_FLOP / (kms * 1e-3)
+        life = steps_done[removed]
         out[prec] = {"value": rate, "unit": "particle-steps/s", "ms_per_solve": wall * 1e3,
                      "kernel_ms": kms, "particle_steps": int(st.particle_steps),
-                     "terminated": int(np.sum(ret == 1)),
+                     "particle_steps_from_closing_records": int(np.sum(steps_done)),
+                     "removed": {"count": int(removed.sum()), "fraction": float(removed.mean()),
+                                 "lifetime_steps": ({"min": int(life.min()), "median": float(np.median(life)),
+                                                     "max": int(life.max())} if life.size else None),
+                                 "rule": "closing record before t1 (retcode stays Success with "
+                                         "save_everystep = false, boris.jl:523-551)"},
                      "roofline": {"bound": "valu", "achieved": flops / 1e12, "peak": peak / 1e12,
                                   "unit": "TFLOP/s", "frac": flops / peak,
                                   "kernel": "k_fixed<S64A, Boris>" if prec == "fp64"
@@ -271,16 +429,33 @@ def bench_dipole(args, torch, TP, W, DeviceEnsemble):
             out[prec]["roofline"]["frac_of_measured_peak"] = flops / meas
         log(f"[bench] dipole {prec}: {rate:.3e} particle-steps/s ({flops / 1e12:.1f} TFLOP/s)")
         del ens
-    u64, r64 = fin["fp64"]
-    u32, r32 = fin["fp32c"]
-    both = (r64 == 0) & (r32 == 0)
+    u64, rm64 = fin["fp64"]
+    u32, rm32 = fin["fp32c"]
+    both = ~rm64 & ~rm32                      # compared only where both precisions survive
+    out["removed_in_one_precision_only"] = int(np.sum(rm64 != rm32))
     v0 = np.sum(w.xv0[:, 3:] ** 2, axis=1)
     for prec, (u, _r) in fin.items():
         drift = np.abs(np.sum(u[:, 3:] ** 2, axis=1) - v0) / v0
         out[prec]["energy_drift"] = {"median": float(np.median(drift[both])),
                                      "max": float(np.max(drift[both]))}
     out["fp32c_vs_fp64"] = rel_pos_err(u32[both, :3], u64[both, :3])
+    out["fp32c_vs_fp64"]["particles"] = int(both.sum())
     out["fp32c_over_fp64"] = out["fp32c"]["value"] / out["fp64"]["value"]
+    if args.cpu_baseline:
+        import oracle as O
+
+        Bo = O.dipole(w.extra["b0"], w.extra["R"])
+
+        def run(n):
+            r = O.solve(w.xv0[:n], O.zero(), Bo, q2m=w.q2m, tspan=(0.0, 0.0), dt_pp=w.dts[:n],
+                        nsteps_pp=w.nsteps, save_everystep=False, bnd_flags=2, r_in=TP.Re,
+                        nthreads=cpu_threads())
+            return r.elapsed, int(r.steps.sum())
+
+        log("[bench] dipole CPU baseline (oracle)")
+        out["cpu_baseline"] = oracle_rate(run, len(w.xv0), cpu_threads(), args.cpu_target_s,
+                                          f"FP64 Boris, per-particle dt, {w.nsteps} steps, removal "
+                                          f"inside 1 R_E")
     return out
 
 
@@ -332,6 +507,38 @@ def bench_batch(args, torch, TP, W):
         torch.cuda.empty_cache()
     out["fp32c_vs_fp64"] = rel_pos_err(fin["fp32c"], fin["fp64"])
     out["fp32c_over_fp64"] = out["fp32c"]["value"] / out["fp64"]["value"]
+    if args.cpu_baseline:
+        import oracle as O
+
+        def run(frac):
+            # the first ceil(frac N_k) particles of every instance, each
+            # instance solved by the oracle on its own grid (fill-NaN
+            # interpolant, exits at the grid's node box)
+            el, steps, n = 0.0, 0, 0
+            for A, axes, xv in inst:
+                k = max(1, int(math.ceil(frac * len(xv))))
+                data = np.ascontiguousarray(A.transpose(3, 2, 1, 0))
+                org = [a[0] for a in axes]
+                sp = [(a[-1] - a[0]) / (len(a) - 1) for a in axes]
+                g = O.grid(data, org, sp, O.EXTRAP_FILL_NAN)
+                hi = [org[d] + float(len(axes[d]) - 1) / (1.0 / sp[d]) for d in range(3)]
+                r = O.solve(xv[:k], O.zero(), g, q2m=TP.qi / TP.mi, tspan=tspan, adaptive=True,
+                            safety=0.1, max_steps=2000, nout_cap=4, save_everystep=False,
+                            bnd_flags=8, box_lo=org, box_hi=hi, nthreads=cpu_threads())
+                el += r.elapsed
+                steps += int(r.steps.sum())
+                n += k
+            return el, steps, n
+
+        log("[bench] batch CPU baseline (oracle)")
+        el, steps, n = run(1e-4)
+        frac = min(1.0, 1e-4 * args.cpu_target_s / max(el, 1e-6))
+        el, steps, n = run(frac)
+        out["cpu_baseline"] = {"value": steps / el, "unit": "particle-steps/s", "cores": cpu_threads(),
+                               "kind": "port", "host": host_info(), "particles_per_s": n / el,
+                               "sample": f"the first {frac:.2e} of each instance's particles: {n} of "
+                                         f"{N}, {steps} particle-steps, FP64 AdaptiveBoris(0.1), "
+                                         f"{el:.1f} s wall"}
     return out
 
 
@@ -339,8 +546,11 @@ def bench_batch(args, torch, TP, W):
 # secondary: adjoint cost and gradient check
 # ---------------------------------------------------------------------------
 def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
+    import oracle as O
+
     n, L, B0, N, nsteps, K = 64, 1.0e7, 5.0e-9, args.adjoint_particles, 2048, 64
     A = W.turbulent_field(n, L, B0, 0.3, 55, dtype=np.float64)
+    A32 = A.astype(np.float32).astype(np.float64)       # the values float storage holds
     ax = W.grid_axes(n, L)
     rng = np.random.default_rng(56)
     x = rng.uniform(0, L, size=(N, 3))
@@ -348,75 +558,128 @@ def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
     xv0 = np.ascontiguousarray(np.hstack([x, v]))
     tg = rng.uniform(0, L, size=(N, 3))
     dt = 2 * math.pi * TP.mi / (TP.qi * B0) / 40
-    tspan = (0.0, nsteps * dt)
+    q2m = TP.qi / TP.mi
     log(f"[bench] adjoint: {n}^3 grid, {N} protons x {nsteps} steps, checkpoints every {K}")
 
-    def prob_of(Av, storage, layout="cells"):
+    def prob_of(Av, storage, h=nsteps, layout="cells"):
         B = TP.GridField(Av, *ax, bc=TP.PeriodicBC(), storage=storage, layout=layout)
-        return TP.TraceProblem(xv0, tspan, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+        return TP.TraceProblem(xv0, (0.0, h * dt), (q2m, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+
+    def grad(prec, h):
+        storage = "f32" if prec == "fp32c" else "f64"
+        return TP.loss_gradient(prob_of(A, storage, h), dt=dt, targets=tg, precision=prec, ckpt_interval=K)
 
     out = {"workload": "adjoint_turbulent_64^3", "particles": N, "steps": nsteps,
            "checkpoint_interval": K, "dt": "T_g(B0)/40", "box": "1e7 m periodic",
-           "loss": "mean |x(T) - x*|^2"}
+           "loss": "mean |x(T) - x*|^2",
+           "precisions": {"fp32c": "float storage, forward / recompute / reverse steps in float, FP64 "
+                                   "checkpoints, float adjoint state with a per-particle power-of-two "
+                                   "exponent, dL/dB summed in double",
+                          "fp64": "double storage and arithmetic"}}
     for prec, storage in (("fp32c", "f32"), ("fp64", "f64")):
         # plain forward solve of the same problem, best of both layouts
         fwd = {}
         for layout in ("nodes", "cells"):
-            ens = DeviceEnsemble(prob_of(A, storage, layout), TP.Boris(), xv0, dt=dt,
+            ens = DeviceEnsemble(prob_of(A, storage, layout=layout), TP.Boris(), xv0, dt=dt,
                                  save_everystep=False, save_start=False, precision=prec)
             _w, kms, _st = timed_runs(ens.run, torch, 1, args.aux_steps)
             fwd[layout] = kms
             del ens
-        pr = prob_of(A, storage)
-        rs = [TP.loss_gradient(pr, dt=dt, targets=tg, precision=prec, ckpt_interval=K)
-              for _ in range(1 + args.aux_steps)][1:]
+        rs = []
+        err = None
+        for _ in range(1 + args.aux_steps):
+            try:
+                rs.append(grad(prec, nsteps))
+            except FloatingPointError as e:          # reported, never returned as a result
+                err = str(e)
+                break
+        if err:
+            out[prec] = {"error": err}
+            continue
+        rs = rs[1:]
         f_ms = float(np.mean([r.forward_ms for r in rs]))
         r_ms = float(np.mean([r.reverse_ms for r in rs]))
         best = min(fwd.values())
         g = rs[-1]
         out[prec] = {"forward_solve_ms": fwd, "adjoint_forward_ms": f_ms, "adjoint_reverse_ms": r_ms,
                      "cost_over_forward": (f_ms + r_ms) / best, "loss": g.loss,
-                     "max_abs_dL_dB": float(np.nanmax(np.abs(g.dL_dB))),
-                     "max_abs_dL_dxv0": float(np.nanmax(np.abs(g.dL_dxv0))),
+                     "max_abs_dL_dB": float(np.max(np.abs(g.dL_dB))),
+                     "max_abs_dL_dxv0": float(np.max(np.abs(g.dL_dxv0))),
                      "nonfinite_gradients": int(np.sum(~np.isfinite(g.dL_dB)) + np.sum(~np.isfinite(g.dL_dxv0)))}
         log(f"[bench] adjoint {prec}: forward solve {best:.2f} ms, adjoint {f_ms:.2f} + {r_ms:.2f} ms "
             f"= {(f_ms + r_ms) / best:.2f}x")
 
-    # Gradient check against central differences of the FP64 device solve on
-    # 32 nodes.  Over 2,048 steps this flow amplifies perturbations by many
-    # orders of magnitude (the exact gradient grows like exp(lambda T)), so the
-    # check runs over one checkpoint segment, where the linearisation holds.
-    hsteps = K
-    tspan_h = (0.0, hsteps * dt)
-
-    def prob_h(Av, storage):
-        B = TP.GridField(Av, *ax, bc=TP.PeriodicBC(), storage=storage)
-        return TP.TraceProblem(xv0, tspan_h, (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
-
-    def loss64(Av):
-        xT = TP.solve(prob_h(Av, "f64"), TP.Boris(), dt=dt, save_everystep=False,
-                      save_start=False).final_states()[:, :3]
-        return float(np.mean(np.sum((xT - tg) ** 2, axis=1)))
-
-    gh = {prec: TP.loss_gradient(prob_h(A, storage), dt=dt, targets=tg, precision=prec, ckpt_interval=16)
-          for prec, storage in (("fp32c", "f32"), ("fp64", "f64"))}
-    g64 = gh["fp64"].dL_dB
-    cand = np.argwhere(np.abs(g64) > 1e-3 * np.abs(g64).max())
-    picks = cand[np.random.default_rng(57).choice(len(cand), 32, replace=False)]
-    h = 1e-6 * B0
-    fd = []
-    for c, i, j, k in picks:
-        Ap, Am = A.copy(), A.copy()
-        Ap[c, i, j, k] += h
-        Am[c, i, j, k] -= h
-        fd.append((loss64(Ap) - loss64(Am)) / (2 * h))
-    fd = np.array(fd)
-    for prec in ("fp32c", "fp64"):
-        ga = np.array([gh[prec].dL_dB[c, i, j, k] for c, i, j, k in picks])
-        rel = np.abs(ga - fd) / np.abs(fd)
-        out[prec]["dL_dB_vs_central_fd"] = {"steps": hsteps, "nodes": 32, "h": h,
-                                            "median_rel": float(np.median(rel)),
-                                            "max_rel": float(np.max(rel))}
+    # Gradient accuracy by horizon (steps), for both precisions:
+    #   dxv0_vs_duals: dL/d(x0, v0) of 64 sampled particles against forward-mode
+    #     duals through the oracle (max over particles of |g - g*| / |g*|);
+    #   dB_dir_vs_duals: <dL/dB, d> along two dense random directions against
+    #     the oracle's duals over all particles (max relative error);
+    #   dB_vs_fd: 32 touched grid values against central differences of the
+    #     FP64 solve, h = 1e-6 B0, formed per particle (median / max relative).
+    #     The FP64 adjoint equals the duals (above), so its column is the error
+    #     of the differences themselves: past ~200 steps the chaotic flow is
+    #     too nonlinear for any difference step, and only the duals pin the
+    #     gradient.
+    # FP32C is compared with the duals of the float-stored field values.
+    grids = {"fp64": O.grid(np.ascontiguousarray(A.transpose(3, 2, 1, 0)), [a[0] for a in ax],
+                            [a[1] - a[0] for a in ax], O.EXTRAP_PERIODIC),
+             "fp32c": O.grid(np.ascontiguousarray(A32.transpose(3, 2, 1, 0)), [a[0] for a in ax],
+                             [a[1] - a[0] for a in ax], O.EXTRAP_PERIODIC)}
+    idx = np.linspace(0, N - 1, 64).astype(int)
+    nvals = 3 * n ** 3
+    dirs = np.random.default_rng(60).normal(size=(2, nvals))
+    acc = {"fp32c": {}, "fp64": {}}
+    nth = cpu_threads()
+    for H in args.adjoint_horizons:
+        tspan = (0.0, H * dt)
+        gs = {}
+        for prec in ("fp32c", "fp64"):
+            try:
+                gs[prec] = grad(prec, H)
+            except FloatingPointError as e:
+                acc[prec][str(H)] = {"error": str(e)}
+        picks = None
+        fd = None
+        if H <= args.adjoint_fd_max and "fp64" in gs:
+            gabs = np.abs(gs["fp64"].dL_dB)
+            cand = np.argwhere(gabs > 1e-3 * gabs.max())
+            picks = cand[np.random.default_rng(57).choice(len(cand), 32, replace=False)]
+            hB = 1e-6 * B0
+            fd = []
+            for c, i, j, k in picks:
+                xs = []
+                for sgn in (1.0, -1.0):
+                    Ap = A.copy()
+                    Ap[c, i, j, k] += sgn * hB
+                    xs.append(TP.solve(prob_of(Ap, "f64", H), TP.Boris(), dt=dt, save_everystep=False,
+                                       save_start=False).final_states()[:, :3])
+                fd.append(float(np.mean(np.sum((xs[0] - xs[1]) * (xs[0] + xs[1] - 2 * tg), axis=1)))
+                          / (2 * hB))
+            fd = np.array(fd)
+        for prec, r in gs.items():
+            row = {}
+            Bo = grids[prec]
+            ref = np.zeros((len(idx), 6))
+            for q, i in enumerate(idx):
+                xT, J = O.boris_tangent(xv0[i], Bo, q2m=q2m, tspan=tspan, dt=dt)
+                ref[q] = 2.0 / N * (xT - tg[i]) @ J
+            e = np.linalg.norm(r.dL_dxv0[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            row["dxv0_vs_duals"] = {"median": float(np.median(e)), "max": float(np.max(e))}
+            if prec == "fp64" or H <= 256:
+                gnode = r.dL_dB.transpose(3, 2, 1, 0).reshape(-1)
+                _l, dd = O.loss_tangent(xv0, tg, Bo, q2m=q2m, tspan=tspan, dt=dt, dense=dirs, nthreads=nth)
+                row["dB_dir_vs_duals_max"] = float(np.max(np.abs(dirs @ gnode - dd) / np.abs(dd)))
+            if fd is not None:
+                ga = np.array([r.dL_dB[c, i, j, k] for c, i, j, k in picks])
+                rel = np.abs(ga - fd) / np.abs(fd)
+                row["dB_vs_fd"] = {"median": float(np.median(rel)), "max": float(np.max(rel)), "nodes": 32,
+                                   "h": 1e-6 * B0}
+            row["max_abs_dL_dB"] = float(np.max(np.abs(r.dL_dB)))
+            acc[prec][str(H)] = row
+            log(f"[bench] adjoint {prec} {H} steps: {json.dumps(row)}")
+    for prec in acc:
+        if prec in out and "error" not in out[prec]:
+            out[prec]["accuracy_by_horizon"] = acc[prec]
     return out
 
 
@@ -473,16 +736,25 @@ def main():
     ap.add_argument("--layout", choices=("cells", "nodes"), default="cells",
                     help="grid storage on the device (cells: one cache line per gather)")
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--cpu-target-s", type=float, default=8.0,
+                    help="seconds of oracle work per secondary section's CPU baseline")
     ap.add_argument("--no-ceiling", dest="ceiling", action="store_false",
                     help="skip the measured gather ceiling")
     ap.add_argument("--no-accuracy", dest="accuracy", action="store_false",
                     help="skip the FP32C-vs-FP64 horizon runs")
+    ap.add_argument("--variants", type=lambda t: [v for v in t.split(",") if v], default=None,
+                    help="headline variants besides the value's (fp32c_reference, fp64_rational, "
+                         "fp32c_rational; default all)")
     ap.add_argument("--sections", default=",".join(SECTIONS),
                     help="comma-separated subset of " + ",".join(SECTIONS))
     ap.add_argument("--aux-steps", type=int, default=2, help="timed runs per secondary section")
     ap.add_argument("--dipole-particles", type=int, default=1 << 20)
     ap.add_argument("--batch-total", type=int, default=20_000_000)
     ap.add_argument("--adjoint-particles", type=int, default=1 << 18)
+    ap.add_argument("--adjoint-horizons", type=lambda t: [int(x) for x in t.split(",") if x],
+                    default=[16, 64, 256, 2048], help="steps at which gradient accuracy is reported")
+    ap.add_argument("--adjoint-fd-max", type=int, default=256,
+                    help="longest horizon with a central-difference check")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the ensemble pusher runs on a single GPU "
@@ -516,12 +788,14 @@ def main():
         # one cell-record gather per particle-step
         out["roofline"]["measured_gather_ceiling"] = ceiling
         out["roofline"]["frac_of_measured_ceiling"] = {
-            "fp32c": out["value"] / ceiling["float_cell_gathers_per_s"],
-            "fp64": out["fp64_value"] / ceiling["double_cell_gathers_per_s"]}
+            f"{k}": v["value"] / ceiling["float_cell_gathers_per_s" if v["storage"] == "f32"
+                                         else "double_cell_gathers_per_s"]
+            for k, v in out["variants"].items()}
     if out:
         log("[bench] headline " + json.dumps(out))
     sec = {}
-    runners = {"dipole": lambda: bench_dipole(args, torch, TP, W, DeviceEnsemble),
+    runners = {"gyration": lambda: bench_gyration(args, torch, TP, W, DeviceEnsemble),
+               "dipole": lambda: bench_dipole(args, torch, TP, W, DeviceEnsemble),
                "batch": lambda: bench_batch(args, torch, TP, W),
                "adjoint": lambda: bench_adjoint(args, torch, TP, W, DeviceEnsemble)}
     for name in SECTIONS[1:]:
