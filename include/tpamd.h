@@ -288,6 +288,39 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
                       int32_t ckpt_interval, double* loss, double* dL_dB, double* dL_dxv0,
                       tpamd_stats* stats);
 
+/* ---- AdaptiveHybrid full-orbit segments ----------------------------------- *
+ * The full-orbit (Boris) branch of the hybrid guiding-centre / full-orbit
+ * solver, _hybrid_adaptive! (src/hybrid.jl:257-452): entering full orbit
+ * (dt = clamp(2 pi safety_fo / (|q2m| |B(x, t)|), dtmin, dtmax) and the back
+ * half-step at t, :262-267 / :298-304), the fixed-dt leapfrog steps with the
+ * final step truncated at t_end (:411-438), and the synchronised state the
+ * adiabaticity check (:361-364) and the closing record (:441-452) use.  The
+ * guiding-centre branch and every mode decision stay with the caller, which
+ * calls this for the particles it keeps in full orbit.
+ *   desc: alg BORIS, precision FP64, q2m, t1 = t_end, safety = safety_fo,
+ *     maxiters (the solver's step cap), save_everystep, savestepinterval,
+ *     the isoutside menu (bnd_flags ...), nout_cap = record capacity per
+ *     particle for this call; dt and t0 are not used.
+ *   Host buffers, per particle i:
+ *     state[i][8]  in/out: x (3), v (3; staggered by -dt/2 once entered), t, dt
+ *     iters[i][2]  in/out: the solver's counters it and steps
+ *     flags[i]     in: TPAMD_FO_ENTER (enter full orbit first), TPAMD_FO_CHECKED
+ *                  (the check at this it is done); out: TPAMD_FO_CHECK (stopped
+ *                  at the next it % check_interval == 0), TPAMD_FO_END (t >= t_end
+ *                  or steps >= maxiters) or TPAMD_FO_OUTSIDE (isoutside fired:
+ *                  x and t kept, v the updated one, as :428-430 leaves them)
+ *     sync[i][6]   out: (x, U(v, x, dt/2, t)) at the particle's t
+ *     out_u[i][cap][6], out_t[i][cap], out_len[i]: the records this call saved
+ *     (out_len may exceed cap: records past cap are dropped and stats->max_len
+ *     says how many were due). */
+enum { TPAMD_FO_ENTER = 1, TPAMD_FO_CHECKED = 2 };
+enum { TPAMD_FO_CHECK = 0, TPAMD_FO_END = 1, TPAMD_FO_OUTSIDE = 2 };
+int32_t tpamd_fo_segment(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* E,
+                         const tpamd_field* B, int64_t N, double dtmin, double dtmax,
+                         int64_t check_interval, double* state, int64_t* iters, int32_t* flags,
+                         double* sync, double* out_u, double* out_t, int32_t* out_len,
+                         tpamd_stats* stats);
+
 /* ---- batched ensembles ---------------------------------------------------- *
  * Many field instances with particle populations of uneven size and lifetime
  * in one persistent launch.  Instance k owns particles [offsets[k],

@@ -83,6 +83,10 @@ def lib():
         L.orc_multistep_coeffs.argtypes = [C.c_double, C.c_int, C.c_int, d3]
         L.orc_rational_polys.argtypes = [C.c_int, d3, d3, d3, d3, d3, C.POINTER(C.c_int)]
         L.orc_boris_tangent.argtypes = [C.POINTER(OrcDesc), C.POINTER(OrcField), d3, d3, d3]
+        L.orc_fo_segment.argtypes = [C.POINTER(OrcDesc), C.POINTER(OrcField), C.POINTER(OrcField),
+                                     C.c_long, C.c_double, C.c_double, C.c_long, d3,
+                                     C.POINTER(C.c_long), C.POINTER(C.c_int), d3, d3, d3,
+                                     C.POINTER(C.c_int)]
         L.orc_fdlibm_atan.argtypes = [C.c_double]
         L.orc_fdlibm_atan.restype = C.c_double
         L.orc_fdlibm_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -396,3 +400,29 @@ def fdlibm_sincos(x: float):
     s, c = C.c_double(), C.c_double()
     lib().orc_fdlibm_sincos(float(x), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def fo_segment(E: Field, B: Field, state, iters, flags, *, q2m, t_end, safety_fo, dtmin, dtmax,
+               check_interval, maxiters=10000, savestepinterval=1, save_everystep=True, cap=16,
+               bnd_flags=0, r_in=0.0, r_out=0.0, box_lo=(-np.inf,) * 3, box_hi=(np.inf,) * 3):
+    """The hybrid solver's full-orbit branch from each particle's state to its
+    next check point, t_end or isoutside (src/hybrid.jl:257-452); state (N, 8),
+    iters (N, 2) int64 and flags (N,) int32 are updated in place.  Returns
+    (sync (N, 6), records (N, cap, 6), times (N, cap), lengths (N,))."""
+    d = OrcDesc(alg=ALG_BORIS, q2m=q2m, t1=t_end, safety=safety_fo, maxiters=maxiters,
+                savestepinterval=savestepinterval, save_everystep=int(save_everystep),
+                nout_cap=cap, bnd_flags=bnd_flags, r_in=r_in, r_out=r_out)
+    d.box_lo[:] = [float(a) for a in box_lo]
+    d.box_hi[:] = [float(a) for a in box_hi]
+    N = state.shape[0]
+    sync = np.zeros((N, 6))
+    ou = np.zeros((N, cap, 6))
+    ot = np.zeros((N, cap))
+    ln = np.zeros(N, dtype=np.int32)
+    rc = lib().orc_fo_segment(C.byref(d), C.byref(E.st), C.byref(B.st), N, dtmin, dtmax,
+                              check_interval, _dp(state), iters.ctypes.data_as(C.POINTER(C.c_long)),
+                              flags.ctypes.data_as(C.POINTER(C.c_int)), _dp(sync), _dp(ou), _dp(ot),
+                              ln.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc:
+        raise ValueError(ERR_NAMES.get(rc, str(rc)))
+    return sync, ou, ot, ln

@@ -1133,6 +1133,85 @@ static void solve_adaptive_one(const ctx_t* c, long i, const double* xv0, double
 }
 
 /* ----------------------------------------------------------------------------
+ * The full-orbit branch of the hybrid solver, _hybrid_adaptive!
+ * (src/hybrid.jl:257-452): from each particle's state, run until its next
+ * adiabaticity check point, t_end, the step cap or its isoutside; the caller
+ * makes the mode decisions.  Buffers as tpamd_fo_segment documents them.
+ * ------------------------------------------------------------------------- */
+static inline double jl_clamp(double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); }
+
+int orc_fo_segment(const orc_desc* d, const orc_field* E, const orc_field* B, long N, double dtmin,
+                   double dtmax, long check_interval, double* state, long* iters, int* flags,
+                   double* sync, double* out_u, double* out_t, int* out_len)
+{
+    if (d->alg != ORC_ALG_BORIS || check_interval < 1 || d->savestepinterval < 1) return ORC_ERR_ARG;
+    const ctx_t c = {d, E, B};
+    const long cap = d->nout_cap, K = d->savestepinterval;
+    const double t_end = d->t1;
+    for (long i = 0; i < N; ++i) {
+        double* st = state + 8 * i;
+        double x[3] = {st[0], st[1], st[2]}, v[3] = {st[3], st[4], st[5]}, vt[3];
+        double t = st[6], dt = st[7];
+        long it = iters[2 * i], steps = iters[2 * i + 1];
+        int checked = (flags[i] & 2) != 0;
+        if (flags[i] & 1) {                                          /* :262-267, :298-304 */
+            double Bv[3];
+            orc_field_eval(B, x, t, Bv);
+            const double om = fabs(d->q2m) * sqrt(sumsq3(Bv));        /* _fo_dt, :244-249 */
+            dt = jl_clamp((2.0 * M_PI) * d->safety / om, dtmin, dtmax);
+            updater(&c, v, x, -0.5 * dt, t, vt);
+            memcpy(v, vt, sizeof v);
+        }
+        int status;
+        long nrec = 0;
+        for (;;) {
+            if (!(t < t_end && steps < d->maxiters)) {              /* :339 */
+                status = 1;
+                break;
+            }
+            if (it % check_interval == 0 && !checked) {             /* :359 */
+                status = 0;
+                break;
+            }
+            checked = 0;
+            if (t + dt > t_end) dt = t_end - t;                      /* :412-414 */
+            const double vprev[3] = {v[0], v[1], v[2]};             /* :418 */
+            updater(&c, vprev, x, dt, t + 0.5 * dt, v);             /* :419 */
+            double xn[3];
+            for (int k = 0; k < 3; ++k) xn[k] = x[k] + v[k] * dt;   /* :420 */
+            const double tn = t + dt;                                /* :421 */
+            if (outside(d, xn, v)) {                                 /* :422-424 */
+                status = 2;
+                break;
+            }
+            if (d->save_everystep && it - 1 > 0 && (it - 1) % K == 0) {    /* :426-430 */
+                updater(&c, vprev, x, 0.5 * dt, t, vt);
+                if (nrec < cap) record(&c, out_u, out_t, cap, i, nrec, x, vt, t);
+                nrec += 1;
+            }
+            memcpy(x, xn, sizeof x);                                 /* :432-435 */
+            t = tn;
+            it += 1;
+            steps += 1;
+        }
+        updater(&c, v, x, 0.5 * dt, t, vt);                          /* :361-364, :446-449 */
+        for (int k = 0; k < 3; ++k) {
+            sync[6 * i + k] = x[k];
+            sync[6 * i + 3 + k] = vt[k];
+            st[k] = x[k];
+            st[3 + k] = v[k];
+        }
+        st[6] = t;
+        st[7] = dt;
+        iters[2 * i] = it;
+        iters[2 * i + 1] = steps;
+        flags[i] = status;
+        out_len[i] = (int)nrec;
+    }
+    return ORC_OK;
+}
+
+/* ----------------------------------------------------------------------------
  * Forward-mode derivative of the final position of a fixed-step standard
  * Boris solve (E = 0, B gridded) with respect to the initial state (x0, v0):
  * the same operation sequence as solve_fixed_one, carried on dual numbers

@@ -149,6 +149,15 @@ int orc_boris_tangent(const orc_desc* d, const orc_field* B, const double xv0[6]
  * NULL, else the single values flat[ndir] = 3 node + c (distinct).  Particles
  * are split into nthreads contiguous chunks; *loss = L, dloss[k] = dL . dir_k. */
 #define ORC_TANGENT_MAX 32
+/* The full-orbit branch of the hybrid solver (src/hybrid.jl:257-452), with
+ * the buffers and status codes of tpamd_fo_segment: state[N][8] (x, v, t,
+ * dt), iters[N][2] (it, steps), flags[N] (in: 1 enter, 2 checked; out: 0
+ * check due, 1 end, 2 outside), sync[N][6], records out_u[N][cap][6],
+ * out_t[N][cap], out_len[N].  d: q2m, t1 = t_end, safety = safety_fo,
+ * maxiters, save_everystep, savestepinterval, nout_cap, the boundary menu. */
+int orc_fo_segment(const orc_desc* d, const orc_field* E, const orc_field* B, long N, double dtmin,
+                   double dtmax, long check_interval, double* state, long* iters, int* flags,
+                   double* sync, double* out_u, double* out_t, int* out_len);
 /* atan and sin/cos of the multistep reference form (FDLIBM, shared with the
  * device library's tpamd_fdlibm.h) */
 double orc_fdlibm_atan(double x);

@@ -1231,6 +1231,113 @@ int32_t tpamd_adjoint(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_
                           f32, bf32, per, cells);
 }
 
+int32_t tpamd_fo_segment(tpamd_ctx* ctx, const tpamd_solve_desc* desc, const tpamd_field* E,
+                         const tpamd_field* B, int64_t N, double dtmin, double dtmax,
+                         int64_t check_interval, double* state, int64_t* iters, int32_t* flags,
+                         double* sync, double* out_u, double* out_t, int32_t* out_len,
+                         tpamd_stats* stats)
+{
+    if (!ctx) return fail(nullptr, TPAMD_ERR_ARGUMENT, "null context");
+    if (!desc) return fail(ctx, TPAMD_ERR_ARGUMENT, "null descriptor");
+    if (N < 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "N must be >= 0");
+    const int64_t cap = desc->nout_cap;
+    if (cap < 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "nout_cap must be >= 0");
+    if (N > 0 && (!state || !iters || !flags || !sync || !out_len || (cap > 0 && (!out_u || !out_t))))
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "null buffer");
+    if (desc->alg != TPAMD_ALG_BORIS || desc->adaptive || desc->precision != TPAMD_FP64)
+        return fail(ctx, TPAMD_ERR_UNSUPPORTED, "full-orbit segments run the FP64 Boris update (src/hybrid.jl:419)");
+    if (desc->save_fields || desc->save_work)
+        return fail(ctx, TPAMD_ERR_UNSUPPORTED, "full-orbit segments save (x, v) records");
+    if (check_interval < 1) return fail(ctx, TPAMD_ERR_ARGUMENT, "check_interval must be positive.");
+    if (desc->savestepinterval < 1) return fail(ctx, TPAMD_ERR_ARGUMENT, "savestepinterval must be >= 1");
+    if (!(std::fabs(desc->q2m) > 0.0) || !std::isfinite(desc->t1))
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "full-orbit segments need a nonzero q/m and a finite t_end");
+    const auto t0 = std::chrono::steady_clock::now();
+    // the parameter block of a one-step solve ending at t_end, then the
+    // segment's own settings
+    tpamd_solve_desc dd = *desc;
+    dd.t0 = desc->t1 - 1.0;
+    dd.dt = 1.0;
+    dd.dt_pp = nullptr;
+    dd.maxiters = 1;
+    dd.save_start = dd.save_end = dd.save_everystep = 0;
+    dd.nout_cap = 0;
+    KParams p;
+    int64_t nt = 0, nout = 0;
+    int rc = fill_params(ctx, &dd, E, B, nullptr, N, nullptr, nullptr, nullptr, nullptr, p, nt, nout);
+    if (rc) return rc;
+    p.t1 = desc->t1;
+    p.cap = cap;
+    p.K = desc->savestepinterval;
+    p.save_every = desc->save_everystep != 0;
+    p.rec_w = 6;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_st = sizeof(double) * 8 * (size_t)N, b_it = sizeof(int64_t) * 2 * (size_t)N,
+                 b_i = sizeof(int32_t) * (size_t)N, b_sy = sizeof(double) * 6 * (size_t)N,
+                 b_u = sizeof(double) * 6 * (size_t)cap * (size_t)N, b_t = sizeof(double) * (size_t)cap * (size_t)N;
+    const size_t need = al(b_st) + al(b_it) + 2 * al(b_i) + al(b_sy) + al(b_u) + al(b_t);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (need > ctx->d_buf_bytes) {
+        if (ctx->d_buf) HIP_TRY(ctx, hipFree(ctx->d_buf));
+        ctx->d_buf = nullptr;
+        ctx->d_buf_bytes = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_buf, need));
+        ctx->d_buf_bytes = need;
+    }
+    char* b = (char*)ctx->d_buf;
+    FoArgs a;
+    a.state = (double*)b; b += al(b_st);
+    a.iters = (long long*)b; b += al(b_it);
+    a.flags = (int*)b; b += al(b_i);
+    int32_t* d_len = (int32_t*)b; b += al(b_i);
+    a.sync = (double*)b; b += al(b_sy);
+    double* d_u = (double*)b; b += al(b_u);
+    double* d_t = (double*)b;
+    a.dtmin = dtmin;
+    a.dtmax = dtmax;
+    a.safety = desc->safety;
+    a.check_interval = check_interval;
+    a.maxiters = desc->maxiters;
+    p.out_u = d_u;
+    p.out_t = d_t;
+    p.out_len = d_len;
+    hipStream_t st = ctx->stream;
+    float ms = 0.f;
+    int64_t ml = 0;
+    if (N > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(a.state, state, b_st, hipMemcpyHostToDevice, st));
+        HIP_TRY(ctx, hipMemcpyAsync(a.iters, iters, b_it, hipMemcpyHostToDevice, st));
+        HIP_TRY(ctx, hipMemcpyAsync(a.flags, flags, b_i, hipMemcpyHostToDevice, st));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+        launch_fo_segment(p, a, solve_class(p), st);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+        HIP_TRY(ctx, hipMemcpyAsync(state, a.state, b_st, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(iters, a.iters, b_it, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(flags, a.flags, b_i, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(sync, a.sync, b_sy, hipMemcpyDeviceToHost, st));
+        if (b_u) HIP_TRY(ctx, hipMemcpyAsync(out_u, d_u, b_u, hipMemcpyDeviceToHost, st));
+        if (b_t) HIP_TRY(ctx, hipMemcpyAsync(out_t, d_t, b_t, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(out_len, d_len, b_i, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ml = max_len_host(out_len, N);
+    }
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->max_len = ml;
+        stats->particle_steps = 0;
+        stats->nt = 0;
+        stats->nout = 0;
+        stats->aux_ms = 0.0;
+    }
+    if (ml > cap)
+        return fail(ctx, TPAMD_ERR_CAPACITY, "a full-orbit segment saved %lld records for a capacity of %lld",
+                    (long long)ml, (long long)cap);
+    return TPAMD_OK;
+}
+
 int32_t tpamd_batch_free(tpamd_batch* b)
 {
     if (!b) return TPAMD_OK;

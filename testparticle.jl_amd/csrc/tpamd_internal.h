@@ -111,6 +111,16 @@ void launch_spec32(const LaunchReq& r, int emode, int ns, bool per, bool cells);
 // the reference-form multistep classes (ns = kRefBase + n)
 void launch_spec64r(const LaunchReq& r, int emode, int ns, bool per, bool f32, bool cells);
 void launch_spec32r(const LaunchReq& r, int emode, int ns, bool per, bool cells);
+// full-orbit segments of the hybrid solver (tpamd_hybrid.hip)
+struct FoArgs {
+    double* state;       // [N][8] x, v, t, dt
+    long long* iters;    // [N][2] it, steps
+    int* flags;          // [N] in: TPAMD_FO_ENTER | TPAMD_FO_CHECKED; out: status
+    double* sync;        // [N][6]
+    double dtmin, dtmax, safety;
+    long long check_interval, maxiters;
+};
+void launch_fo_segment(const KParams& p, const FoArgs& a, int fk, hipStream_t st);
 // the checkpointed adjoint (tpamd_adj.hip)
 int adjoint_launch(tpamd_ctx* ctx, const KParams& p, const tpamd_field* B, const double* xv0,
                    int64_t N, const double* targets, int K, int64_t nt, double* loss,

@@ -24,6 +24,7 @@ from .problem import (Boundary, EnsembleContext, InsideSphere, OutsideBox, Outsi
 from .solution import EnsembleSolution, TraceSolution
 from .solve import initial_states, make_desc, plan, solve, solve_batch
 from .adjoint import AdjointResult, loss_gradient, trajectory_loss
+from .hybrid import AdaptiveHybrid, FullOrbitResult, TraceHybridProblem, fo_segment, full_orbit, hybrid
 from .postprocess import get_fields, get_work, record_eval
 from .wrapper import TraceResult, trace, trace_fieldline, trace_gc
 

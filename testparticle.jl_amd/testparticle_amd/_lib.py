@@ -28,6 +28,8 @@ FORM_REFERENCE, FORM_RATIONAL = 0, 1
 FP64, FP32C = 0, 1
 BND_NAN, BND_SPHERE_IN, BND_SPHERE_OUT, BND_BOX, BND_GRID = 1, 2, 4, 8, 16
 RET_SUCCESS, RET_TERMINATED, RET_MAXITERS = 0, 1, 2
+FO_ENTER, FO_CHECKED = 1, 2                     # tpamd_fo_segment flags in
+FO_CHECK, FO_END, FO_OUTSIDE = 0, 1, 2          # and status out
 FLAG_GENERIC_KERNELS = 1
 RETCODE_NAMES = {RET_SUCCESS: "Success", RET_TERMINATED: "Terminated", RET_MAXITERS: "MaxIters"}
 
@@ -83,7 +85,7 @@ EXPORTS = [
     "tpamd_field_free", "tpamd_record_eval",
     "tpamd_plan", "tpamd_solve", "tpamd_solve_device", "tpamd_adjoint",
     "tpamd_batch_create", "tpamd_batch_autotune", "tpamd_batch_run", "tpamd_batch_free",
-    "tpamd_solve_batch",
+    "tpamd_solve_batch", "tpamd_fo_segment",
 ]
 
 _lib = None
@@ -156,6 +158,9 @@ def load_library(path: str = LIB_PATH):
         L.tpamd_batch_free.argtypes = [vp]
         L.tpamd_solve_batch.argtypes = [vp, C.POINTER(SolveDesc), C.c_int32, vpa, vpa,
                                         C.POINTER(i64), vp, vp, vp, vp, vp, C.POINTER(Stats)]
+        L.tpamd_fo_segment.argtypes = [vp, C.POINTER(SolveDesc), vp, vp, i64, C.c_double, C.c_double,
+                                       i64, d3, C.POINTER(i64), C.POINTER(C.c_int32), d3, d3, d3,
+                                       C.POINTER(C.c_int32), C.POINTER(Stats)]
         for name in EXPORTS:
             if name not in ("tpamd_abi_version", "tpamd_last_error"):
                 getattr(L, name).restype = C.c_int32
@@ -168,6 +173,14 @@ def load_library(path: str = LIB_PATH):
 
 def dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def iptr64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
 
 
 class Context:
