@@ -352,6 +352,28 @@ int32_t tpamd_solve_batch(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t 
                           double* out_t, int32_t* out_len, int32_t* retcode,
                           tpamd_stats* stats);
 
+/* ---- allocation-free batched solve ----------------------------------------- *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the batched solve of tpamd_batch_create + tpamd_batch_run
+ * (same kernels, same results, device buffers as there) as one call that
+ * calls no hipMalloc / hipFree.  Its scratch (queue claim counters, step
+ * counter, the per-instance parameter table, the chunk list) is a fixed
+ * 3.4 MiB device array each GPU holds until the process ends, so:
+ *   - at most 256 instances and 131,072 chunks of 64 x particles_per_lane
+ *     particles (particles_per_lane from desc, default 8: ~67M particles);
+ *     larger batches return TPAMD_ERR_CAPACITY;
+ *   - calls that overlap on one GPU (two host threads, or two streams in
+ *     flight) race on that array; serialise them.
+ * No graph is captured and nothing is autotuned (desc->block_size, default
+ * 256).  Host memory for the table is ordinary heap memory.  Returns after
+ * the work on `stream` (NULL = the context's) has completed. */
+int32_t tpamd_solve_batch_noalloc(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t n_inst,
+                                  const tpamd_field* const* E, const tpamd_field* const* B,
+                                  const int64_t* offsets, const double* d_xv0, double* d_out_u,
+                                  double* d_out_t, int32_t* d_out_len, int32_t* d_retcode,
+                                  void* stream, tpamd_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -485,21 +485,29 @@ struct tpamd_batch {
 
 namespace {
 
-int batch_chunks(tpamd_batch* b, int ppt, int stride)
+// the work chunks of a batch: 64 ppt consecutive particles of one instance,
+// every stride-th of them
+std::vector<Chunk> make_chunks(const std::vector<int64_t>& offsets, int n_inst, int ppt, int stride)
 {
     std::vector<Chunk> ch;
     const int64_t csz = 64 * (int64_t)ppt;
     int64_t k = 0;
-    for (int i = 0; i < b->n_inst; ++i)
-        for (int64_t s = b->offsets[i]; s < b->offsets[i + 1]; s += csz, ++k)
+    for (int i = 0; i < n_inst; ++i)
+        for (int64_t s = offsets[i]; s < offsets[i + 1]; s += csz, ++k)
             if (k % stride == 0) {
                 Chunk c;
                 c.inst = i;
                 c.pad = 0;
                 c.begin = s;
-                c.end = std::min(s + csz, b->offsets[i + 1]);
+                c.end = std::min(s + csz, offsets[i + 1]);
                 ch.push_back(c);
             }
+    return ch;
+}
+
+int batch_chunks(tpamd_batch* b, int ppt, int stride)
+{
+    const std::vector<Chunk> ch = make_chunks(b->offsets, b->n_inst, ppt, stride);
     tpamd_ctx* ctx = b->ctx;
     if (ch.size() > b->chunks_cap) {
         if (b->d_chunks) HIP_TRY(ctx, hipFree(b->d_chunks));
@@ -591,7 +599,26 @@ int batch_time_once(tpamd_batch* b, hipStream_t st, float* ms)
 }
 
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free batched solve (tpamd_solve_batch_noalloc):
+// the kQueues claim counters and the step counter, the per-instance
+// parameter table and the chunk list, in one file-scope device array of
+// fixed size instead of memory the library allocates.  Each GPU holds its
+// own copy; tpamd_init looks up its address on the context's device.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxInst = 256;
+constexpr long long kNoAllocMaxChunks = 131072;
+constexpr size_t kNoAllocCounterBytes = 4096;
+constexpr size_t kNoAllocTableBytes = (sizeof(KParams) * kNoAllocMaxInst + 255) & ~(size_t)255;
+constexpr size_t kNoAllocBytes =
+    kNoAllocCounterBytes + kNoAllocTableBytes + sizeof(Chunk) * (size_t)kNoAllocMaxChunks;
+static_assert(sizeof(unsigned) * kQueues * kQueueStride + sizeof(unsigned long long) <= kNoAllocCounterBytes,
+              "claim counters and the step counter fit their region");
+
 }  // namespace
+
+// 3.4 MiB: counters, a 256-instance table, 131,072 chunks
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 extern "C" {
 
@@ -611,12 +638,19 @@ int32_t tpamd_init(int32_t device, tpamd_ctx** out)
     c->device = device;
     c->d_buf = nullptr;
     c->d_buf_bytes = 0;
+    c->scratch = nullptr;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipMalloc((void**)&c->d_steps, sizeof(unsigned long long)) != hipSuccess) {
         delete c;
         return fail(nullptr, TPAMD_ERR_DEVICE, "HIP context set-up failed on device %d", device);
     }
+    // this device's copy of the allocation-free entry point's scratch array
+    // (with the device current); the name is a host-side stand-in, only the
+    // looked-up address may reach a kernel or a copy
+    void* sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess) c->scratch = (unsigned char*)sp;
+    else (void)hipGetLastError();
     *out = c;
     return TPAMD_OK;
 }
@@ -1434,6 +1468,106 @@ int32_t tpamd_batch_create(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t
         return rc;
     }
     *out = b;
+    return TPAMD_OK;
+}
+
+int32_t tpamd_solve_batch_noalloc(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t n_inst,
+                                  const tpamd_field* const* E, const tpamd_field* const* B,
+                                  const int64_t* offsets, const double* d_xv0, double* d_out_u,
+                                  double* d_out_t, int32_t* d_out_len, int32_t* d_retcode,
+                                  void* stream, tpamd_stats* stats)
+{
+    if (!ctx) return fail(nullptr, TPAMD_ERR_ARGUMENT, "null context");
+    if (!desc || !E || !B || !offsets || n_inst < 1)
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "batch needs a descriptor, n_inst >= 1 fields and offsets");
+    if (n_inst > kNoAllocMaxInst)
+        return fail(ctx, TPAMD_ERR_CAPACITY, "the allocation-free batch takes at most %d instances (got %d)",
+                    kNoAllocMaxInst, n_inst);
+    if (desc->dt_pp) return fail(ctx, TPAMD_ERR_UNSUPPORTED, "per-particle dt is not supported in batches");
+    if (offsets[0] != 0) return fail(ctx, TPAMD_ERR_ARGUMENT, "offsets[0] must be 0");
+    for (int i = 0; i < n_inst; ++i)
+        if (offsets[i + 1] < offsets[i]) return fail(ctx, TPAMD_ERR_ARGUMENT, "offsets must be non-decreasing");
+    const int64_t N = offsets[n_inst];
+    if (N > 0 && (!d_xv0 || !d_out_len || !d_retcode || (desc->nout_cap > 0 && (!d_out_u || !d_out_t))))
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "null buffer");
+    if (!ctx->scratch)
+        return fail(ctx, TPAMD_ERR_DEVICE, "the scratch array's device address was not found at tpamd_init");
+    const int bs = desc->block_size > 0 ? desc->block_size : 256;
+    if (bs % 64 != 0 || bs > 256) return fail(ctx, TPAMD_ERR_ARGUMENT, "block_size must be a multiple of 64 up to 256");
+    const int ppt = desc->particles_per_lane > 0 ? desc->particles_per_lane : 8;
+    if (ppt > 1024) return fail(ctx, TPAMD_ERR_ARGUMENT, "particles_per_lane must be in [0, 1024]");
+    // host-side set-up, as tpamd_batch_create does, on a batch object whose
+    // device pointers are regions of the scratch array
+    tpamd_batch b;
+    b.ctx = ctx;
+    b.desc = *desc;
+    b.n_inst = n_inst;
+    b.N = N;
+    b.offsets.assign(offsets, offsets + n_inst + 1);
+    b.table.resize(n_inst);
+    b.block_size = bs;
+    b.ppt = ppt;
+    b.have_graph = false;
+    int64_t nt, nout;
+    for (int i = 0; i < n_inst; ++i) {
+        const int rc = fill_params(ctx, desc, E[i], B[i], d_xv0, N, d_out_u, d_out_t, d_out_len, d_retcode,
+                                   b.table[i], nt, nout);
+        if (rc) return rc;
+    }
+    int em0 = 0;
+    const int ns0 = spec_class(desc, b.table[0], &em0);
+    b.allow_spec = ns0 >= 0;
+    b.fk = kFkAnalytic;
+    for (int i = 0; i < n_inst; ++i) b.fk = std::max(b.fk, solve_class(b.table[i]));
+    for (int i = 1; i < n_inst && b.allow_spec; ++i) {
+        int em = 0;
+        const int ns = spec_class(desc, b.table[i], &em);
+        b.allow_spec = ns == ns0 && em == em0 && b.table[i].B.dtype == b.table[0].B.dtype &&
+                       (b.table[i].B.extrap == kExtrapPeriodic) == (b.table[0].B.extrap == kExtrapPeriodic) &&
+                       (b.table[i].B.cells != nullptr) == (b.table[0].B.cells != nullptr);
+    }
+    const std::vector<Chunk> ch = make_chunks(b.offsets, n_inst, ppt, 1);
+    if ((long long)ch.size() > kNoAllocMaxChunks)
+        return fail(ctx, TPAMD_ERR_CAPACITY,
+                    "the allocation-free batch takes at most %lld chunks of 64 x particles_per_lane particles "
+                    "(got %lld at particles_per_lane %d): raise particles_per_lane or split the batch",
+                    kNoAllocMaxChunks, (long long)ch.size(), ppt);
+    b.nchunks = (int)ch.size();
+    b.chunks_cap = 0;
+    unsigned char* sp = ctx->scratch;
+    b.d_counter = (unsigned*)sp;
+    b.d_steps = (unsigned long long*)(sp + sizeof(unsigned) * kQueues * kQueueStride);
+    b.d_table = (KParams*)(sp + kNoAllocCounterBytes);
+    b.d_chunks = (Chunk*)(sp + kNoAllocCounterBytes + kNoAllocTableBytes);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int n_cu = 0;
+    HIP_TRY(ctx, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    b.n_cu = n_cu;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    // the table and the chunks into the scratch array; the counters are
+    // cleared by the launch's own reset kernel (no memset node)
+    HIP_TRY(ctx, hipMemcpyAsync(b.d_table, b.table.data(), sizeof(KParams) * n_inst, hipMemcpyHostToDevice, st));
+    if (!ch.empty())
+        HIP_TRY(ctx, hipMemcpyAsync(b.d_chunks, ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    const int rc = batch_enqueue(&b, st);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    unsigned long long steps = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&steps, b.d_steps, sizeof steps, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->particle_steps = (int64_t)steps;
+        stats->nt = b.table[0].nt;
+        stats->nout = b.table[0].nout;
+        stats->max_len = -1;
+        stats->aux_ms = 0.0;
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
     return TPAMD_OK;
 }
 

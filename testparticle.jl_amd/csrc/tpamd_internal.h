@@ -30,6 +30,9 @@ struct tpamd_ctx {
     void* d_buf;
     size_t d_buf_bytes;
     unsigned long long* d_steps;
+    // the device address of this device's copy of the allocation-free
+    // entry point's scratch array (looked up at tpamd_init; null if that failed)
+    unsigned char* scratch;
 };
 
 namespace tpamd {

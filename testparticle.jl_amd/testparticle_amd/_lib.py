@@ -85,7 +85,7 @@ EXPORTS = [
     "tpamd_field_free", "tpamd_record_eval",
     "tpamd_plan", "tpamd_solve", "tpamd_solve_device", "tpamd_adjoint",
     "tpamd_batch_create", "tpamd_batch_autotune", "tpamd_batch_run", "tpamd_batch_free",
-    "tpamd_solve_batch", "tpamd_fo_segment",
+    "tpamd_solve_batch", "tpamd_fo_segment", "tpamd_solve_batch_noalloc",
 ]
 
 _lib = None
@@ -158,6 +158,9 @@ def load_library(path: str = LIB_PATH):
         L.tpamd_batch_free.argtypes = [vp]
         L.tpamd_solve_batch.argtypes = [vp, C.POINTER(SolveDesc), C.c_int32, vpa, vpa,
                                         C.POINTER(i64), vp, vp, vp, vp, vp, C.POINTER(Stats)]
+        L.tpamd_solve_batch_noalloc.argtypes = [vp, C.POINTER(SolveDesc), C.c_int32, vpa, vpa,
+                                                C.POINTER(i64), vp, vp, vp, vp, vp, vp,
+                                                C.POINTER(Stats)]
         L.tpamd_fo_segment.argtypes = [vp, C.POINTER(SolveDesc), vp, vp, i64, C.c_double, C.c_double,
                                        i64, d3, C.POINTER(i64), C.POINTER(C.c_int32), d3, d3, d3,
                                        C.POINTER(C.c_int32), C.POINTER(Stats)]

@@ -144,3 +144,48 @@ class DeviceBatch:
         import sys
         if not sys.is_finalizing():
             self.close()
+
+
+def solve_batch_noalloc(problems, alg, *, device: int = 0, nout_cap=None, particles_per_lane: int = 0,
+                        **desc_kw):
+    """One batched solve through tpamd_solve_batch_noalloc: the kernels of
+    DeviceBatch, with the library's scratch in its fixed device array, so the
+    call allocates no device memory (the output tensors here are torch's).
+    Returns (u, t, lengths, retcodes) device tensors, the offsets and the
+    stats."""
+    import torch
+
+    from .solve import _adaptive_cap, initial_states
+
+    ctx = _lib.context(device)
+    dev = torch.device(f"cuda:{device}")
+    fields = [_fields(pr) for pr in problems]
+    states = [initial_states(pr, 1) for pr in problems]
+    offsets = np.zeros(len(problems) + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum([s.shape[0] for s in states])
+    N = int(offsets[-1])
+    desc = make_desc(problems[0], alg, **desc_kw)
+    if alg.adaptive:
+        cap = nout_cap if nout_cap is not None else _adaptive_cap(
+            desc_kw.get("save_start", True), desc_kw.get("save_everystep", True),
+            desc_kw.get("max_steps", 0), desc_kw.get("savestepinterval", 1))
+    else:
+        cap = plan(desc)[1]
+    desc.nout_cap = int(cap)
+    desc.particles_per_lane = int(particles_per_lane)
+    xv0 = torch.from_numpy(np.ascontiguousarray(np.concatenate(states))).to(dev)
+    u = torch.empty((N, max(int(cap), 1), desc.record_width), dtype=torch.float64, device=dev)
+    t = torch.empty((N, max(int(cap), 1)), dtype=torch.float64, device=dev)
+    ln = torch.empty(N, dtype=torch.int32, device=dev)
+    ret = torch.empty(N, dtype=torch.int32, device=dev)
+    hE = (C.c_void_p * len(problems))(*[f[0].handle(ctx).value for f in fields])
+    hB = (C.c_void_p * len(problems))(*[f[1].handle(ctx).value for f in fields])
+    st = _lib.Stats()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with ctx.lock:
+        rc = ctx.L.tpamd_solve_batch_noalloc(
+            ctx.h, C.byref(desc), len(problems), hE, hB, offsets.ctypes.data_as(C.POINTER(C.c_int64)),
+            xv0.data_ptr(), u.data_ptr(), t.data_ptr(), ln.data_ptr(), ret.data_ptr(), C.c_void_p(stream),
+            C.byref(st))
+    ctx.check(rc, st)
+    return u, t, ln, ret, offsets, st
