@@ -309,17 +309,18 @@ def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
                        "kernel_ms": rv["kernel_ms"],
                        "algorithmic_bytes_per_particle_step": GATHER_BYTES["f64"],
                        "algorithmic_bytes_per_launch": bytes_launch}
-    # HBM traffic per launch: FETCH_SIZE + WRITE_SIZE per particle-step from the
-    # rocprofv3 --pmc passes recorded under profiles/ (scaled by the copy-probe
-    # calibration), times this launch's particle-steps
+    # HBM traffic per launch from the rocprofv3 --pmc passes recorded under
+    # profiles/ (scaled by the copy-probe calibration): FETCH_SIZE per
+    # particle-step times this launch's particle-steps, plus WRITE_SIZE per
+    # particle (the two records) times its particles
     prof = os.path.join(ROOT, "profiles", f"traffic_turbulent_box_{n}.json")
     if os.path.exists(prof):
         with open(prof) as f:
             t = json.load(f)
         v = t.get("variants", {}).get(f"{vkey[0]}_{vkey[1]}")
         if t.get("layout") == args.layout and v:
-            out["roofline"]["traffic"] = ps * (v["fetch_bytes_per_particle_step"] +
-                                               v["write_bytes_per_particle_step"])
+            out["roofline"]["traffic"] = (ps * v["fetch_bytes_per_particle_step"] +
+                                          N * v["write_bytes_per_particle"])
             out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
     if args.cpu_baseline:
         log("[bench] CPU baseline (oracle, host cores)")
