@@ -52,6 +52,9 @@ GATHER_BYTES = {"f32": 8 * 3 * 4, "f64": 8 * 3 * 8}
 # per particle of a solve: its initial state (6 doubles) read, its length and
 # return code (2 x int32) written; per record: x, v (6 doubles) and t
 STATE_BYTES, LEN_BYTES, RECORD_BYTES = 6 * 8, 2 * 4, 7 * 8
+# adjoint reverse sweep per particle-step: 2 gathers + segment state written
+# and read + dL/dB record (key + Bbar + weights) written, read, written, read
+ADJ_REVERSE_BYTES = {"fp32c": 2 * 96 + 2 * 36 + 4 * 32, "fp64": 2 * 192 + 2 * 48 + 4 * 52}
 # algorithmic FLOPs per particle-step (FMA = 2; div, sqrt = 1): standard Boris
 # with E = 0 (37) + position update (6) + dipole evaluation (27)
 DIPOLE_FLOP = 37 + 6 + 27
@@ -487,9 +490,11 @@ def bench_batch(args, torch, TP, W):
         fin[prec] = b.final_states()[:, :3]
         rate = st.particle_steps / wall
         # particles live ~1.4 steps, so the per-particle state and record
-        # bytes weigh as much as the per-step gathers (float grids: 96 B)
+        # bytes weigh as much as the per-step gathers (float grids: 96 B).
+        # AdaptiveBoris gathers once at the start (dt from |B(x0)|) and once
+        # after every completed step: particle_steps + particles gathers.
         nrec = int(b.len.sum().item())
-        algo = (st.particle_steps * GATHER_BYTES["f32"] + N * (STATE_BYTES + LEN_BYTES)
+        algo = ((st.particle_steps + N) * GATHER_BYTES["f32"] + N * (STATE_BYTES + LEN_BYTES)
                 + nrec * RECORD_BYTES)
         achieved = algo / (kms * 1e-3)
         out[prec] = {"value": rate, "unit": "particle-steps/s", "ms_per_run": wall * 1e3,
@@ -500,8 +505,9 @@ def bench_batch(args, torch, TP, W):
                                   "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                                   "frac": achieved / HBM_PEAK, "kernel": "k_batch (adaptive)",
                                   "algorithmic_bytes_per_launch": int(algo),
-                                  "bytes": "96 B per step (float cell gather) + 56 B per particle "
-                                           "(state in, length and return code) + 56 B per record"}}
+                                  "bytes": "96 B per gather (float cell; particle_steps + particles "
+                                           "gathers) + 56 B per particle (state in, length and return "
+                                           "code) + 56 B per record"}}
         log(f"[bench] batch {prec}: {rate:.3e} particle-steps/s, block {bs}, ppt {ppt}")
         b.close()
         del b
@@ -602,8 +608,28 @@ def bench_adjoint(args, torch, TP, W, DeviceEnsemble):
         r_ms = float(np.mean([r.reverse_ms for r in rs]))
         best = min(fwd.values())
         g = rs[-1]
+        # the reverse sweep's roofline: per particle-step two gathers
+        # (recompute and reverse), the segment state written and read back,
+        # and the dL/dB record written, sorted (read + write) and summed (read)
+        abytes = ADJ_REVERSE_BYTES[prec]
+        psteps = N * nsteps
+        roof = {"bound": "hbm", "kernel": "reverse sweep: k_adj_reverse + k_adj_localsort + k_adj_bins_seg",
+                "achieved": psteps * abytes / (r_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": psteps * abytes / (r_ms * 1e-3) / HBM_PEAK,
+                "algorithmic_bytes_per_particle_step": abytes, "traffic": None,
+                "forward_frac": psteps * GATHER_BYTES["f32" if prec == "fp32c" else "f64"]
+                                / (best * 1e-3) / HBM_PEAK}
+        prof = os.path.join(ROOT, "profiles", "traffic_adjoint_turbulent_64.json")
+        if os.path.exists(prof):
+            with open(prof) as f:
+                t = json.load(f)
+            roof["traffic"] = psteps * t["kernels"][prec]["total_bytes_per_particle_step"]
+            roof["traffic_source"] = os.path.relpath(prof, ROOT)
         out[prec] = {"forward_solve_ms": fwd, "adjoint_forward_ms": f_ms, "adjoint_reverse_ms": r_ms,
-                     "cost_over_forward": (f_ms + r_ms) / best, "loss": g.loss,
+                     "cost_over_forward": (f_ms + r_ms) / best, "roofline": roof,
+                     "cost_floor_by_bytes": (GATHER_BYTES["f32" if prec == "fp32c" else "f64"] + abytes)
+                                            / GATHER_BYTES["f32" if prec == "fp32c" else "f64"],
+                     "loss": g.loss,
                      "max_abs_dL_dB": float(np.max(np.abs(g.dL_dB))),
                      "max_abs_dL_dxv0": float(np.max(np.abs(g.dL_dxv0))),
                      "nonfinite_gradients": int(np.sum(~np.isfinite(g.dL_dB)) + np.sum(~np.isfinite(g.dL_dxv0)))}

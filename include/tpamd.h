@@ -135,7 +135,8 @@ typedef struct tpamd_solve_desc {
     int32_t save_work;        /* records gain (P_par, P_fermi, P_grad, P_betatron) */
     int32_t particles_per_lane; /* adaptive solves only: each wave works through 64 x this many
                                    particles, a lane taking the next one when its own ends;
-                                   0 = library choice (a batch has its own, tpamd_batch_autotune) */
+                                   0 = library choice; a batch starts from this (0: 4) until
+                                   tpamd_batch_autotune picks its own */
     int32_t reserved;
 } tpamd_solve_desc;
 

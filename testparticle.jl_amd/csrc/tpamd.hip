@@ -1418,11 +1418,15 @@ int32_t tpamd_batch_create(tpamd_ctx* ctx, const tpamd_solve_desc* desc, int32_t
     b->d_counter = nullptr;
     b->d_steps = nullptr;
     b->block_size = desc->block_size > 0 ? desc->block_size : 256;
-    b->ppt = 4;
+    b->ppt = desc->particles_per_lane > 0 ? desc->particles_per_lane : 4;
     b->have_graph = false;
     if (b->block_size % 64 != 0 || b->block_size > 256) {
         tpamd_batch_free(b);
         return fail(ctx, TPAMD_ERR_ARGUMENT, "block_size must be a multiple of 64 up to 256");
+    }
+    if (desc->particles_per_lane < 0 || desc->particles_per_lane > 1024) {
+        tpamd_batch_free(b);
+        return fail(ctx, TPAMD_ERR_ARGUMENT, "particles_per_lane must be in [0, 1024]");
     }
     int64_t nt, nout;
     for (int i = 0; i < n_inst; ++i) {

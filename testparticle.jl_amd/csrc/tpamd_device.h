@@ -24,6 +24,27 @@
 namespace tpamd {
 
 constexpr int kRationalNMax = 12;
+
+// Every buffer the kernels read or write is device (global) memory.  A pointer
+// the batch kernel takes from its per-instance parameter table is loaded from
+// memory, so the compiler cannot tell its address space and emits flat
+// accesses; flat accesses count against the scalar loads' wait counter too, so
+// each wait for a parameter would also wait for every gather and record store
+// in flight.  Accesses through gbl() compile to global loads and stores (the
+// host pass, which only checks these bodies, keeps the plain pointer).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gbl(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+#else
+template <class T>
+__device__ __forceinline__ T* gbl(T* p)
+{
+    return p;
+}
+#endif
 // adaptive solves without a step cap stop after this many steps per particle
 constexpr long long kAdaptiveHardCap = 1LL << 31;
 

@@ -89,9 +89,9 @@ __device__ __forceinline__ bool locate_f32(const DevField& f, int d, float s, in
 }
 
 template <typename T>
-__device__ __forceinline__ const T* at(const void* base, unsigned off)
+__device__ __forceinline__ auto at(const void* base, unsigned off)
 {
-    return (const T*)((const char*)base + off);
+    return gbl((const T*)((const char*)base + off));
 }
 
 // Trilinear gather over byte offsets; OUT is float or double, lerp chosen to
@@ -104,8 +104,8 @@ __device__ __forceinline__ void trilinear(const void* base, const unsigned ox[2]
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const unsigned oyz = oy[q & 1] + oz[q >> 1];
-        const T* a = at<T>(base, ox[0] + oyz);
-        const T* b = at<T>(base, ox[1] + oyz);
+        const auto a = at<T>(base, ox[0] + oyz);
+        const auto b = at<T>(base, ox[1] + oyz);
         OUT av[3] = {(OUT)a[0], (OUT)a[1], (OUT)a[2]};
         OUT bv[3] = {(OUT)b[0], (OUT)b[1], (OUT)b[2]};
 #pragma unroll
@@ -132,14 +132,14 @@ __device__ __forceinline__ void trilinear_cell(const void* base, unsigned off, c
 {
     T v[24];
     if constexpr (sizeof(T) == 4) {
-        const float4* q = (const float4*)((const char*)base + off);
+        const auto q = gbl((const float4*)((const char*)base + off));
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
             const float4 a = q[j];
             v[4 * j] = a.x; v[4 * j + 1] = a.y; v[4 * j + 2] = a.z; v[4 * j + 3] = a.w;
         }
     } else {
-        const double2* q = (const double2*)((const char*)base + off);
+        const auto q = gbl((const double2*)((const char*)base + off));
 #pragma unroll
         for (int j = 0; j < 12; ++j) {
             const double2 a = q[j];

@@ -39,7 +39,7 @@ struct S64T {
 
     __device__ __forceinline__ void load(const KParams& p, long long i)
     {
-        const double* s = p.xv0 + 6 * i;
+        const auto s = gbl(p.xv0) + 6 * i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) { pos.x[c] = s[c]; v[c] = s[3 + c]; }
     }
@@ -98,8 +98,16 @@ struct S32CT {
     float v[3];
     float dt_f, dt_hi, dt_lo;
 
-    __device__ __forceinline__ void load(const KParams& p, long long i) { load_from(p, p.xv0 + 6 * i); }
-    __device__ __forceinline__ void load_from(const KParams& p, const double* s)
+    __device__ __forceinline__ void load(const KParams& p, long long i)
+    {
+        const auto g = gbl(p.xv0) + 6 * i;
+        double s[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s[c] = g[c];
+        load_from(p, s);
+    }
+    // the state from x, v values (positions relative to the frame, wrapped)
+    __device__ __forceinline__ void load_from(const KParams& p, const double s[6])
     {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -240,9 +248,9 @@ __device__ __forceinline__ bool outside(const KParams& p, const S& s, const type
 // extra record columns (E, B and/or work rates) at (x, v, t), in FP64.  Only
 // the REC instantiations of the generic kernels contain it, so the pushers
 // that save plain records keep their register budget.
-template <int FK>
+template <int FK, class RP>
 __device__ __forceinline__ void record_extra(const KParams& p, const double x[3], const double v[3],
-                                          double t, double* r)
+                                          double t, RP r)
 {
     double E[3], B[3], J[3][3], dBdt[3];
     field_jac_f64<FK>(p.B, x, t, B, J, dBdt);
@@ -270,11 +278,11 @@ __device__ __forceinline__ void record(const KParams& p, const S& s, long long i
     double x[3], v[3];
     s.abs_pos(p, s.pos, x);
     s.vel_d(vv, v);
-    double* r = p.out_u + (i * p.cap + k) * p.rec_w;
+    const auto r = gbl(p.out_u) + (i * p.cap + k) * p.rec_w;
     r[0] = x[0]; r[1] = x[1]; r[2] = x[2];
     r[3] = v[0]; r[4] = v[1]; r[5] = v[2];
     if constexpr (REC) record_extra<S::kFK>(p, x, v, t, r);
-    p.out_t[i * p.cap + k] = t;
+    gbl(p.out_t)[i * p.cap + k] = t;
 }
 
 template <class S>
@@ -293,7 +301,7 @@ __device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
     using R = typename S::R;
     S s;
     s.load(p, i);
-    const double dt = p.dt_pp ? p.dt_pp[i] : p.dt;
+    const double dt = p.dt_pp ? gbl(p.dt_pp)[i] : p.dt;
     const double t0 = p.t0;
     const double t1 = p.dt_pp ? t0 + (double)p.nt * dt : p.t1;
     s.set_dt(dt);
@@ -356,25 +364,31 @@ __device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
             iout += 1;
         }
     }
-    p.out_len[i] = (int)iout;                                                 // :545-551
-    p.retcode[i] = (iout < p.nout) ? kRetTerminated : kRetSuccess;
+    gbl(p.out_len)[i] = (int)iout;                                            // :545-551
+    gbl(p.retcode)[i] = (iout < p.nout) ? kRetTerminated : kRetSuccess;
     return final_step;
 }
 
 // _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266, as a
-// resumable particle: init (the prologue, :160-195), step (one pass of the
-// loop, :197-248; false once the particle is done) and finish (the closing
-// record, :250-259).  The kernels interleave many particles on one wave,
-// handing a lane a new particle as soon as its previous one is done
-// (adaptive_chunk); each particle still sees the same operations in the same
-// order as when it runs alone.
+// resumable particle in three parts: begin (the prologue up to the first field
+// evaluation, :160-186), then per loop pass settle (the fields at the current
+// (x, t) and the step size they set: the rest of the prologue, :164,
+// :188-192, for a new particle, the end of the previous pass, :237-247,
+// otherwise) and advance (the pass up to the move, :197-235), and finish (the
+// closing record, :250-259).  The kernels interleave many particles on one
+// wave, handing a lane a new particle as soon as its previous one is done
+// (adaptive_chunk); a new particle's first field evaluation and a running
+// particle's resynchronisation are the same code, so a wave issues one
+// gather and one dt computation per pass whatever mix of new and running
+// particles it holds.  Each particle still sees the same operations in the
+// same order as when it runs alone.
 template <class S, int ALG, bool REC>
 struct AdaptiveRun {
     using R = typename S::R;
     S s;
     long long i, it, next_save, nrec, cap_steps;
     double t, dt, tlast;
-    bool should;
+    bool should, fresh;
     int ret;
     // the fields at the current (x, t).  The reference evaluates there up to
     // four times per step (|B| for dt, both halves of the resynchronisation,
@@ -389,21 +403,20 @@ struct AdaptiveRun {
         nrec += 1;
     }
 
-    __device__ __forceinline__ void init(const KParams& p, long long idx)
+    __device__ __forceinline__ void begin(const KParams& p, long long idx)
+    {
+        s.load(p, idx);
+        start(p, idx);
+    }
+    // the prologue with the state already loaded into s
+    __device__ __forceinline__ void start(const KParams& p, long long idx)
     {
         i = idx;
-        s.load(p, i);
         t = p.t0;
         nrec = 0;
         tlast = 0.0;
+        dt = 0.0;
         if (p.save_start) push(p, s.v, t);                   // :179-186
-        s.eval(p, t, Ec, Bc);
-        dt = p.C / s.bmag_of(Bc);                             // :164, :188-189
-        s.set_dt(dt);
-        R vt[3];
-        s.template Uf<ALG>(p, Ec, Bc, s.v, -0.5 * dt, vt);    // :192
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
         it = 1;
         next_save = p.save_every ? p.K + 1 : -1;
         should = p.save_end;
@@ -411,9 +424,37 @@ struct AdaptiveRun {
         // every particle must stop: a zero step (|B| = inf) or an unbounded
         // run ends with MaxIters instead of spinning
         cap_steps = p.max_steps > 0 ? p.max_steps : kAdaptiveHardCap;
+        fresh = true;
     }
 
-    __device__ __forceinline__ bool step(const KParams& p)
+    // false once the particle is done
+    __device__ __forceinline__ bool settle(const KParams& p)
+    {
+        s.eval(p, t, Ec, Bc);
+        const double dtn = p.C / s.bmag_of(Bc);               // :164, :188-189 / :237-238
+        R vt[3];
+        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);     // :241-242 (running particles)
+        if (fresh) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vt[c] = s.v[c];
+        }
+        s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dtn, s.v);   // :192 / :242
+        dt = dtn;
+        s.set_dt(dt);
+        if (fresh) {
+            fresh = false;
+            return true;
+        }
+        it += 1;
+        if (it == next_save) should = true;                   // :245-247, (it-1) % K == 0
+        if (it - 1 >= cap_steps) {
+            ret = kRetMaxIters;
+            return false;
+        }
+        return true;
+    }
+
+    __device__ __forceinline__ bool advance(const KParams& p)
     {
         const double t0 = p.t0, t1 = p.t1, ttotal = t1 - t0;
         if (!(fabs(t - t0) < fabs(ttotal))) return false;    // :197
@@ -450,18 +491,6 @@ struct AdaptiveRun {
         t = tn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.v[c] = vn[c];
-        s.eval(p, t, Ec, Bc);
-        const double dtn = p.C / s.bmag_of(Bc);               // :237-238
-        s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);     // :241-242
-        s.template Uf<ALG>(p, Ec, Bc, vt, -0.5 * dtn, s.v);
-        dt = dtn;
-        s.set_dt(dt);
-        it += 1;
-        if (it == next_save) should = true;                   // :245-247, (it-1) % K == 0
-        if (it - 1 >= cap_steps) {
-            ret = kRetMaxIters;
-            return false;
-        }
         return true;
     }
 
@@ -472,8 +501,8 @@ struct AdaptiveRun {
             s.template Uf<ALG>(p, Ec, Bc, s.v, 0.5 * dt, vt);
             push(p, vt, t);
         }
-        p.out_len[i] = (int)nrec;
-        p.retcode[i] = ret;
+        gbl(p.out_len)[i] = (int)nrec;
+        gbl(p.retcode)[i] = ret;
         return it - 1;
     }
 };
@@ -509,11 +538,11 @@ __device__ __forceinline__ long long adaptive_chunk(const KParams& p, long long 
     long long steps = 0;
     AdaptiveRun<S, ALG, REC> r;
     bool active = begin + lane < end;
-    if (active) r.init(p, begin + lane);
+    if (active) r.begin(p, begin + lane);
     long long next = begin + 64;                          // wave-uniform cursor
     const unsigned long long below = (1ull << lane) - 1ull;
     while (true) {
-        if (active && !r.step(p)) {
+        if (active && !(r.settle(p) && r.advance(p))) {
             steps += r.finish(p);
             active = false;
         }
@@ -525,7 +554,7 @@ __device__ __forceinline__ long long adaptive_chunk(const KParams& p, long long 
         if (!active) {
             const long long idx = next + (long long)__popcll(idle & below);
             if (idx < end) {
-                r.init(p, idx);
+                r.begin(p, idx);
                 active = true;
             }
         }
@@ -660,37 +689,132 @@ struct S32G : S32C {
     }
 };
 
+// The batch kernel's work source: the chunks of the kQueues slices, claimed
+// one at a time by a wave.  The chunk index is made wave-uniform
+// (readfirstlane), so the chunk and its instance's parameter block are read
+// with scalar loads; the next chunk is claimed before the current one is
+// worked on, which hides the atomic's round trip.  A wave starts on slice
+// blockIdx % kQueues and moves to the next slice once its own is drained.
+struct ChunkSource {
+    const Chunk* chunks;
+    unsigned* counter;
+    int nchunks, lane, j;
+    unsigned q0, q, nq, c, pending;
+
+    __device__ __forceinline__ unsigned claim() const
+    {
+        unsigned v = 0;
+        if (lane == 0) v = atomicAdd(counter + q * kQueueStride, 1u);
+        return (unsigned)__builtin_amdgcn_readfirstlane((int)v);
+    }
+    __device__ __forceinline__ void open_slice()
+    {
+        q = (q0 + (unsigned)j) % kQueues;
+        nq = (unsigned)nchunks > q ? ((unsigned)nchunks - q + kQueues - 1) / kQueues : 0u;
+        c = claim();
+        pending = c < nq ? claim() : nq;
+    }
+    __device__ __forceinline__ void init(const Chunk* ch, unsigned* ctr, int n, int ln)
+    {
+        chunks = ch;
+        counter = ctr;
+        nchunks = n;
+        lane = ln;
+        q0 = blockIdx.x % kQueues;
+        j = 0;
+        open_slice();
+    }
+    // the next chunk of this wave, false once every slice is drained
+    __device__ __forceinline__ bool next(Chunk& out)
+    {
+        while (c >= nq) {
+            if (++j >= kQueues) return false;
+            open_slice();
+        }
+        out = chunks[q + c * kQueues];
+        c = pending;
+        pending = c < nq ? claim() : nq;
+        return true;
+    }
+};
+
+// Adaptive particles of every chunk the wave claims: a lane takes the next
+// particle as soon as its own is done (see adaptive_chunk).  Lives are
+// uneven (in a turbulent box most particles leave within a step or two,
+// the longest of a 512-particle chunk lives tens of steps), so when a chunk's
+// last particle has been handed out the wave claims its next chunk at once
+// and keeps feeding idle lanes from it while the old chunk's long-lived
+// particles finish, instead of idling until they do.  That needs the new
+// chunk to be of the same instance (the lanes share one parameter block);
+// a chunk of another instance waits until every lane is done.
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long batch_adaptive(const KParams* __restrict__ table, ChunkSource& src,
+                                                    int lane)
+{
+    long long steps = 0;
+    AdaptiveRun<S, ALG, REC> r;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    Chunk ch, nch;
+    bool have = src.next(ch);
+    while (have) {
+        // one instance at a time: its parameter block stays in scalar
+        // registers (the index is made wave-uniform explicitly)
+        const int inst = __builtin_amdgcn_readfirstlane(ch.inst);
+        const KParams& p = table[inst];
+        long long next = ch.begin, end = ch.end;          // wave-uniform cursor
+        bool held = false;                                // nch: claimed, other instance
+        bool drained = false;                             // no chunk left to claim
+        bool active = false;
+        while (true) {
+            if (active && !(r.settle(p) && r.advance(p))) {
+                steps += r.finish(p);
+                active = false;
+            }
+            const unsigned long long idle = __ballot(!active);
+            while (next >= end && !held && !drained) {
+                if (!src.next(nch)) {
+                    drained = true;
+                } else if (__builtin_amdgcn_readfirstlane(nch.inst) == inst) {
+                    next = nch.begin;
+                    end = nch.end;
+                } else {
+                    held = true;
+                }
+            }
+            if (next >= end) {
+                if (idle != ~0ull) continue;              // running particles still to finish
+                break;
+            }
+            if (!active) {
+                const long long idx = next + (long long)__popcll(idle & below);
+                if (idx < end) {
+                    r.begin(p, idx);
+                    active = true;
+                }
+            }
+            next += (long long)__popcll(idle);
+        }
+        ch = nch;
+        have = held;
+    }
+    return steps;
+}
+
 template <class S, int ALG, bool ADAPT, bool REC = false>
-__global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table,
-                                               const Chunk* __restrict__ chunks, int nchunks,
-                                               unsigned* __restrict__ counter,
-                                               unsigned long long* __restrict__ steps_out)
+__global__ void __launch_bounds__(256) k_batch(const KParams* __restrict__ table, const Chunk* __restrict__ chunks, int nchunks,
+        unsigned* __restrict__ counter, unsigned long long* __restrict__ steps_out)
 {
     const int lane = threadIdx.x & 63;
     long long steps = 0;
-    // the chunk index is made wave-uniform (readfirstlane), so the chunk and
-    // the instance's parameter block are read with scalar loads; the next
-    // chunk is claimed before the current one is worked on, which hides the
-    // atomic's round trip.  Slices: see kQueues.
-    const unsigned q0 = blockIdx.x % kQueues;
-    for (int j = 0; j < kQueues; ++j) {
-        const unsigned q = (q0 + (unsigned)j) % kQueues;
-        const unsigned nq = (unsigned)nchunks > q ? ((unsigned)nchunks - q + kQueues - 1) / kQueues : 0u;
-        unsigned* ctr = counter + q * kQueueStride;
-        unsigned claim = 0;
-        if (lane == 0) claim = atomicAdd(ctr, 1u);
-        unsigned c = (unsigned)__builtin_amdgcn_readfirstlane((int)claim);
-        while (c < nq) {
-            unsigned next = 0;
-            if (lane == 0) next = atomicAdd(ctr, 1u);
-            const Chunk ch = chunks[q + c * kQueues];
+    ChunkSource src;
+    src.init(chunks, counter, nchunks, lane);
+    if constexpr (ADAPT) {
+        steps = batch_adaptive<S, ALG, REC>(table, src, lane);
+    } else {
+        Chunk ch;
+        while (src.next(ch)) {
             const KParams& p = table[ch.inst];
-            if constexpr (ADAPT) {
-                steps += adaptive_chunk<S, ALG, REC>(p, ch.begin, ch.end, lane);
-            } else {
-                for (long long i = ch.begin + lane; i < ch.end; i += 64) steps += run_fixed<S, ALG, REC>(p, i);
-            }
-            c = (unsigned)__builtin_amdgcn_readfirstlane((int)next);
+            for (long long i = ch.begin + lane; i < ch.end; i += 64) steps += run_fixed<S, ALG, REC>(p, i);
         }
     }
     unsigned long long v = (unsigned long long)steps;

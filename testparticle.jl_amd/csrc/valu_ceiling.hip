@@ -9,7 +9,8 @@
 // packed math).  Prints one JSON line: FLOP/s of the FMA chains with an FMA
 // counted as 2, and lane-instructions/s per kind (fma, add, mul, a compare
 // and select pair counted as two instructions, a 32-bit xor-and-add pair,
-// which the compiler fuses into one v_xad_u32).
+// which the compiler fuses into one v_xad_u32, and v_readlane_b32 counted as
+// 64 lane-instructions per wave instruction, as SQ_INSTS_VALU x 64 counts it).
 //
 // The loop count is fixed and nothing is read from memory, so every wave
 // ends; one store per lane keeps the chains live.
@@ -31,7 +32,7 @@ namespace {
 
 constexpr int kChains = 8;
 
-enum Op { kFma, kAdd, kMul, kSel, kInt };
+enum Op { kFma, kAdd, kMul, kSel, kInt, kReadlane };
 
 template <typename T, int OP>
 __device__ __forceinline__ T step(T x, T a, T b)
@@ -46,7 +47,20 @@ template <typename T, int OP>
 __global__ void __launch_bounds__(256) k_op(T a, T b, int iters, T* __restrict__ out)
 {
     const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if constexpr (OP == kInt) {
+    if constexpr (OP == kReadlane) {
+        // v_readlane_b32 (one lane of a VGPR into an SGPR), the instruction
+        // that keeps uniform values in VGPR lanes when SGPRs run out; the
+        // lane index changes every iteration so the reads cannot be hoisted
+        int v[kChains];
+#pragma unroll
+        for (int c = 0; c < kChains; ++c) v[c] = (int)tid * (c + 1);
+        int acc = 0;
+        for (int it = 0; it < iters; ++it) {
+#pragma unroll
+            for (int c = 0; c < kChains; ++c) acc ^= __builtin_amdgcn_readlane(v[c], (it + c) & 63);
+        }
+        out[tid] = (T)acc;
+    } else if constexpr (OP == kInt) {
         unsigned x[kChains];
         const unsigned ia = (unsigned)(tid * 2654435761u);
 #pragma unroll
@@ -114,10 +128,11 @@ int main()
     printf("{\"cu\": %d, \"clock_mhz\": %d, \"fp64_fma_flops_per_s\": %.6e, \"fp32_fma_flops_per_s\": %.6e, "
            "\"lane_instructions_per_s\": {\"fma_f64\": %.6e, \"add_f64\": %.6e, \"mul_f64\": %.6e, "
            "\"fma_f32\": %.6e, \"add_f32\": %.6e, \"mul_f32\": %.6e, \"cmp_sel_f32\": %.6e, "
-           "\"add_u32\": %.6e}}\n",
+           "\"add_u32\": %.6e, \"readlane_b32\": %.6e}}\n",
            prop.multiProcessorCount, prop.clockRate / 1000, 2.0 * f64, 2.0 * f32, f64,
            lane_ops<double, kAdd>(blocks, iters), lane_ops<double, kMul>(blocks, iters), f32,
            lane_ops<float, kAdd>(blocks, iters), lane_ops<float, kMul>(blocks, iters),
-           2.0 * lane_ops<float, kSel>(blocks, iters), lane_ops<float, kInt>(blocks, iters));
+           2.0 * lane_ops<float, kSel>(blocks, iters), lane_ops<float, kInt>(blocks, iters),
+           lane_ops<float, kReadlane>(blocks, iters));
     return 0;
 }
