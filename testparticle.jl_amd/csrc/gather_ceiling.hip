@@ -8,6 +8,9 @@
 // chain of random cell indices over a 2 GiB buffer (the size of the 256^3
 // float cell grid), so the rate it reaches is the ceiling a pusher kernel can
 // approach.  Prints one JSON line: gathers/s and line-equivalent bytes/s.
+// An optional argument sets the buffer size in GiB (1-64): random gathers over
+// more memory than the address translation caches cover run slower, which is
+// what a larger grid (or many grids at once) would meet.
 //
 // Every index is (hash % ncells), so every access stays inside the buffer; the
 // loop count is fixed, so every wave ends.
@@ -85,9 +88,14 @@ double rate(K kern, const float4* buf, unsigned ncells, float* out, int blocks, 
 
 }  // namespace
 
-int main()
+int main(int argc, char** argv)
 {
-    const size_t bytes = (size_t)2 << 30;
+    const int gib = argc > 1 ? atoi(argv[1]) : 2;
+    if (gib < 1 || gib > 64) {
+        fprintf(stderr, "gather_ceiling: buffer size must be 1-64 GiB\n");
+        return 2;
+    }
+    const size_t bytes = (size_t)gib << 30;
     float4* buf = nullptr;
     float* out = nullptr;
     CK(hipMalloc((void**)&buf, bytes));
