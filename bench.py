@@ -243,6 +243,22 @@ def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
                         r.setdefault("final" if tag == "x0" else "final_p", {})[h] = e.final_states()[:, :3]
                         del e
             res[key] = r
+        if prec == "fp64" and args.layout == "cells" and (not args.variants or
+                                                          "fp64_reference_nodes" in args.variants):
+            # the plain FP64 pusher: FP64 on the node layout the caller's array
+            # is in (4-8 lines per gather), reference form
+            Bn = TP.GridField(A, *ax, bc=TP.PeriodicBC(), layout="nodes")
+            pn = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, TP.UxBField(w.extra["u"], Bn), Bn,
+                                                  TP.ZeroField()))
+            log("[bench] fp64 reference, node layout: uploading and warming up")
+            ens = DeviceEnsemble(pn, alg, w.xv0, dt=w.dt, save_everystep=False, precision=prec,
+                                 form="reference")
+            wall, kms, st = timed_runs(ens.run, torch, 1, 1)
+            assert st.particle_steps == N * nsteps, st.particle_steps
+            log(f"[bench] fp64 reference, node layout: {wall * 1e3:.1f} ms/solve")
+            res[("fp64", "reference_nodes")] = dict(wall=wall, kernel_ms=kms, storage=storage, steps=1,
+                                                    final={nsteps: ens.final_states()[:, :3]})
+            del ens, Bn, pn
         del B, E, prob
         torch.cuda.empty_cache()
     ps = N * nsteps
@@ -257,7 +273,9 @@ def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
     for key, r in res.items():
         variants[f"{key[0]}_{key[1]}"] = {
             "value": ps / r["wall"], "ms_per_solve": r["wall"] * 1e3, "kernel_ms": r["kernel_ms"],
-            "timed_solves": r["steps"], "storage": r["storage"], "kernel": _kernel_name(*key, args.layout),
+            "timed_solves": r["steps"], "storage": r["storage"],
+            "kernel": (_kernel_name("fp64", "reference", "nodes") if key[1] == "reference_nodes"
+                       else _kernel_name(*key, args.layout)),
             "algorithmic_bytes_per_particle_step": GATHER_BYTES[r["storage"]],
             "hbm_frac": ps * GATHER_BYTES[r["storage"]] / (r["kernel_ms"] * 1e-3) / HBM_PEAK}
     out = {
@@ -285,6 +303,12 @@ def bench_turbulent(args, torch, TP, W, DeviceEnsemble):
     }
     if ("fp32c", "reference") in res:
         out["fp32c_over_fp64"] = res[("fp64", "reference")]["wall"] / res[("fp32c", "reference")]["wall"]
+        if ("fp64", "reference_nodes") in res:
+            out["fp32c_over_plain_fp64"] = {
+                "value": res[("fp64", "reference_nodes")]["wall"] / res[("fp32c", "reference")]["wall"],
+                "note": "FP32C (float cells) against the plain FP64 pusher: FP64 storage and arithmetic "
+                        "on the node layout the caller's array is in, reference form; fp32c_over_fp64 "
+                        "compares both on the cell layout"}
     if "host_wall" in rv:
         out["host_buffers"] = {"value": ps / rv["host_wall"], "ms_per_solve": rv["host_wall"] * 1e3,
                                "note": "tpamd_solve from host arrays (FP64, reference form): PCIe "
@@ -775,7 +799,7 @@ def main():
                     help="skip the FP32C-vs-FP64 horizon runs")
     ap.add_argument("--variants", type=lambda t: [v for v in t.split(",") if v], default=None,
                     help="headline variants besides the value's (fp32c_reference, fp64_rational, "
-                         "fp32c_rational; default all)")
+                         "fp32c_rational, fp64_reference_nodes; default all; 'none' for none)")
     ap.add_argument("--sections", default=",".join(SECTIONS),
                     help="comma-separated subset of " + ",".join(SECTIONS))
     ap.add_argument("--aux-steps", type=int, default=2, help="timed runs per secondary section")
