@@ -845,7 +845,7 @@ def main():
         out["roofline"]["frac_of_measured_ceiling"] = {
             f"{k}": v["value"] / ceiling["float_cell_gathers_per_s" if v["storage"] == "f32"
                                          else "double_cell_gathers_per_s"]
-            for k, v in out["variants"].items()}
+            for k, v in out["variants"].items() if not k.endswith("_nodes")}   # cell gathers only
     if out:
         log("[bench] headline " + json.dumps(out))
     sec = {}
