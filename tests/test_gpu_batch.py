@@ -90,3 +90,46 @@ def test_batch_graph_replays_and_autotune_are_consistent():
             np.testing.assert_array_equal(b.final_states(), ref)
             np.testing.assert_array_equal(b.len.cpu().numpy(), ref_len)
         b.close()
+
+
+@pytest.mark.parametrize("layout", ["nodes", "cells"])
+@pytest.mark.parametrize("bs,ppt", [(64, 1), (128, 3), (256, 16)])
+def test_batch_chunking_does_not_change_results(layout, bs, ppt):
+    """Block size and chunk size (64 x ppt particles) only change which wave
+    runs which particle: lane refill, a wave taking the next chunk of the same
+    instance while long-lived lanes finish, a chunk of another instance held
+    until the wave is empty.  Every particle's records, lengths and return
+    codes stay bit-identical to the default launch."""
+    pytest.importorskip("torch")
+    from testparticle_amd.device import DeviceBatch
+
+    rng = np.random.default_rng(77)
+    L, B0 = 1.0e7, 5e-9
+    probs = []
+    for k, (n, N) in enumerate(zip([16, 8, 24, 12, 8], [3000, 70, 5000, 1, 900])):
+        A = W.turbulent_field(n, L, B0, 0.5, 700 + k, dtype=np.float32)
+        ax = W.grid_axes(n, L, periodic=False)
+        B = TP.GridField(A, *ax, bc=TP.FillExtrap(), layout=layout)
+        x = rng.uniform(0, L, size=(N, 3))
+        v = W.isotropic_directions(rng, N) * np.sqrt(2 * rng.uniform(1e3, 1e5, N) * TP.qi / TP.mi)[:, None]
+        probs.append(TP.TraceProblem(np.hstack([x, v]), (0.0, 200 * 2 * np.pi * TP.mi / (TP.qi * B0)),
+                                     (TP.qi / TP.mi, TP.mi, TP.ZeroField(), B, TP.ZeroField())))
+    alg = TP.AdaptiveBoris(safety=0.1)
+    for prec in ("fp64", "fp32c"):
+        kw = dict(precision=prec, isoutside=TP.OutsideGrid(), max_steps=2000, save_everystep=False)
+        out = []
+        for extra in ({}, dict(workgroup_size=bs, particles_per_lane=ppt)):
+            b = DeviceBatch(probs, alg, **kw, **extra)
+            st = b.run()
+            ln = b.len.cpu().numpy()
+            keep = np.arange(b.cap)[None, :] < ln[:, None]
+            out.append((st.particle_steps, ln, b.ret.cpu().numpy(), b.u.cpu().numpy()[keep],
+                        b.t.cpu().numpy()[keep]))
+            b.close()
+        (s0, l0, r0, u0, t0), (s1, l1, r1, u1, t1) = out
+        assert s0 == s1 and s0 > 0
+        np.testing.assert_array_equal(l1, l0)
+        np.testing.assert_array_equal(r1, r0)
+        np.testing.assert_array_equal(u1, u0)
+        np.testing.assert_array_equal(t1, t0)
+        assert (r0 == 1).any() and (l0 >= 2).any()
