@@ -99,10 +99,10 @@ def rel_pos_err(xa, xb):
 
 
 def host_info():
-    """The host cores the CPU baselines run on: the threads used (the box
-    grants one GPU's process 16 host cores, so the oracle's OpenMP team is
-    sized to min(16, affinity)), the affinity count, the CPU model and the SMT
-    state."""
+    """The host cores the CPU baselines run on: the threads used (the
+    oracle's OpenMP team is sized to min(16, affinity): 16 host cores is the
+    share of one GPU on the machines this runs on), the affinity count, the
+    CPU model and the SMT state."""
     info = {"threads": cpu_threads()}
     try:
         info["affinity_cpus"] = len(os.sched_getaffinity(0))
