@@ -692,8 +692,10 @@ struct S32G : S32C {
 // The batch kernel's work source: the chunks of the kQueues slices, claimed
 // one at a time by a wave.  The chunk index is made wave-uniform
 // (readfirstlane), so the chunk and its instance's parameter block are read
-// with scalar loads; the next chunk is claimed before the current one is
-// worked on, which hides the atomic's round trip.  A wave starts on slice
+// with scalar loads.  The index of the chunk after the current one is
+// claimed when the current one is taken (reading the claim later, to overlap
+// the atomic with work, kept a value live through the particle loop and cost
+// 15 VGPRs: 6 waves per SIMD down to 5).  A wave starts on slice
 // blockIdx % kQueues and moves to the next slice once its own is drained.
 struct ChunkSource {
     const Chunk* chunks;
