@@ -22,8 +22,9 @@
 //
 // dL/dB without global atomics: each reverse step writes one record (the
 // cell it sampled, Bbar and the three trilinear weights).  Per segment the
-// records are radix-partitioned by cell bin, and one workgroup per bin sums
-// its records into an LDS tile of per-cell corner moments (8 corners x 3
+// records are partitioned by cell bin (a block-local sort in LDS, or a radix
+// sort for grids with more bins than its counters hold), and one workgroup
+// per bin sums its records into per-cell corner moments (8 corners x 3
 // components), which it alone adds into the global per-cell gradient.  A
 // last pass folds the per-cell corner moments into the node gradient.
 #pragma once
