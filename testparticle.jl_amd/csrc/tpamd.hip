@@ -212,6 +212,8 @@ int setup_fields(tpamd_ctx* ctx, const tpamd_field* E, const tpamd_field* B, int
         return f.kind != kFieldAffine || (f.ft[0] == 0.0 && f.ft[1] == 0.0 && f.ft[2] == 0.0);
     };
     p.static_fields = is_static(B->d) && (p.e_is_uxb || is_static(E->d)) ? 1 : 0;
+    auto is_uniform = [](const DevField& f) { return f.kind == kFieldZero || f.kind == kFieldUniform; };
+    p.uniform_fields = is_uniform(B->d) && (p.e_is_uxb || is_uniform(E->d)) ? 1 : 0;
     for (int d = 0; d < 3; ++d) { p.frame_origin[d] = 0.0; p.frame_period[d] = 0.0; }
     p.frame_periodic = 0;
     p.span = 64;              // adaptive single solves set their own

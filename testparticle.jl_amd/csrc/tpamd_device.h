@@ -119,6 +119,8 @@ struct KParams {
     int save_start, save_end, save_every, bnd_flags;
     int e_is_uxb, frame_periodic;
     int static_fields;        // neither E nor B depends on t
+    int uniform_fields;       // E and B are the same at every x and t (zero / uniform, or
+                              // E = -u x B of a uniform B)
     int rec_w;                // record columns: 6, + 6 (E, B), + 4 (work rates)
     int save_fields, save_work;
     int span;                 // adaptive solves: particles per wave (64 x particles per lane)
@@ -184,6 +186,35 @@ __device__ __forceinline__ void boris_f64(const double v[3], const double E[3], 
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = (vm[c] + vps[c]) + h * E[c];
 }
+
+// boris_f64 for fields that are the same at every step: t, s and h E depend
+// only on the fields and h, so they are formed once (by the same operations,
+// so the update is bit-identical) and each step does the rest.
+struct BorisConst64 {
+    double t[3], s[3], hE[3];
+    __device__ __forceinline__ void init(const double E[3], const double B[3], double h)
+    {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = h * B[c];
+        const double den = 1.0 + dot3(t, t);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] = (2.0 * t[c]) / den;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hE[c] = h * E[c];
+    }
+    __device__ __forceinline__ void apply(const double v[3], double out[3]) const
+    {
+        double vm[3], vmt[3], vp[3], vps[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vm[c] = v[c] + hE[c];
+        cross3(vm, t, vmt);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vmt[c];
+        cross3(vp, s, vps);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c] = (vm[c] + vps[c]) + hE[c];
+    }
+};
 
 // Horner over p[0..d], d <= NMAX: r = p[d], then r = fma(r, u, p[j]) for
 // j = d-1 .. 0.  The loop runs NMAX+1 unrolled iterations with static
@@ -685,6 +716,36 @@ __device__ __forceinline__ void boris_f32(const float v[3], const float E[3], co
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = fmaf(h, E[c], vm[c] + vps[c]);
 }
+
+// boris_f32 with t and s formed once (see BorisConst64); h E stays inside
+// the fused multiply-adds, as boris_f32 forms it
+struct BorisConst32 {
+    float t[3], s[3], E[3], h;
+    __device__ __forceinline__ void init(const float Ev[3], const float B[3], float hv)
+    {
+        h = hv;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = h * B[c];
+        const float r = 2.0f / (1.0f + dot3f(t, t));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            s[c] = t[c] * r;
+            E[c] = Ev[c];
+        }
+    }
+    __device__ __forceinline__ void apply(const float v[3], float out[3]) const
+    {
+        float vm[3], vmt[3], vp[3], vps[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vm[c] = fmaf(h, E[c], v[c]);
+        cross3f(vm, t, vmt);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vmt[c];
+        cross3f(vp, s, vps);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c] = fmaf(h, E[c], vm[c] + vps[c]);
+    }
+};
 
 template <int NMAX>
 __device__ __forceinline__ float horner_f32(const double* p, int d, float u)

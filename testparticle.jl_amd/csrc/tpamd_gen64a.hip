@@ -6,6 +6,12 @@ namespace tpamd {
 
 void launch_generic64_analytic(const LaunchReq& r, int alg)
 {
+    // fixed-step Boris in fields that are the same everywhere: the kernel
+    // that forms the update's rotation vectors once per particle
+    if (!r.batch && !r.adaptive && alg == 0 && r.p->uniform_fields && r.p->rec_w == 6) {
+        SoloLaunch{*r.p, 0, r.g, r.b, r.st}.template go<Hoisted<S64A>, 0>();
+        return;
+    }
     with_launcher(r, [&](const auto& l) {
         if (alg == 0) l.template go<S64A, 0>();
         else l.template go<S64A, 1>();

@@ -31,6 +31,7 @@ struct S64T {
     static constexpr int kNS = -1;   // generic: any algorithm, adaptive or not
     static constexpr int kFK = FK;   // field-kind class (kFk*)
     static constexpr int kWaves = 1; // waves per SIMD the fixed-step kernel must fit (1: no bound)
+    static constexpr bool kHoist = false;
     using R = double;
     struct Pos { double x[3]; };
     Pos pos;
@@ -92,6 +93,7 @@ struct S32CT {
     static constexpr int kNS = -1;
     static constexpr int kFK = FK;
     static constexpr int kWaves = 1;
+    static constexpr bool kHoist = false;
     using R = float;
     using Pos = Pos32;
     Pos pos;
@@ -198,6 +200,15 @@ using S32CA = S32CT<kFkAnalytic>;
 using S32C = S32CT<kFkGrid>;
 using S32CM = S32CT<kFkMesh>;
 
+// a policy whose fixed-step Boris kernel forms the update's rotation vectors
+// once per particle (run_fixed_impl's HOIST): for solves whose fields are the
+// same at every x and t; a kernel of its own, so that the analytic-class
+// kernels keep their register counts
+template <class Base>
+struct Hoisted : Base {
+    static constexpr bool kHoist = true;
+};
+
 template <class S>
 __device__ __forceinline__ bool outside(const KParams& p, const S& s, const typename S::Pos& xn,
                                         const typename S::R vv[3])
@@ -293,10 +304,29 @@ __device__ __forceinline__ void do_step(const KParams& p, const S& s, const type
     else s.step_impl(p, vv, xn);
 }
 
+// the per-step update of fields that are the same everywhere, formed once
+template <class S, bool F64 = sizeof(typename S::R) == 8>
+struct BorisConst : BorisConst64 {
+    __device__ __forceinline__ void setup(const KParams& p, const double E[3], const double B[3], double dt)
+    {
+        init(E, B, (p.q2m * 0.5) * dt);                      // as S64T::Uf forms h
+    }
+};
+template <class S>
+struct BorisConst<S, false> : BorisConst32 {
+    __device__ __forceinline__ void setup(const KParams& p, const float E[3], const float B[3], double dt)
+    {
+        init(E, B, (float)((p.q2m * 0.5) * dt));             // as S32CT::Uf forms h
+    }
+};
+
 // _boris_single + _boris_loop!, src/boris/boris.jl:445-557.  The save test
 // `(it - 1) % K == 0` is carried as a countdown to the next save step.
-template <class S, int ALG, bool REC>
-__device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
+// HOIST (Boris, fields the same at every x and t): the fields are evaluated
+// once and the rotation vectors of the step's update formed once per
+// particle; every step computes exactly what the plain update computes.
+template <class S, int ALG, bool REC, bool HOIST>
+__device__ __forceinline__ long long run_fixed_impl(const KParams& p, long long i)
 {
     using R = typename S::R;
     S s;
@@ -321,12 +351,21 @@ __device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
     long long it = 1;
     long long next_save = p.save_every ? p.K + 1 : nt + 2;    // first it with (it-1) % K == 0, it-1 > 0
     double t = fma(-0.5, dt, t0);                             // :451
+    R Ec[3], Bc[3];
+    BorisConst<S> bc;
+    if constexpr (HOIST) {
+        s.eval(p, t0, Ec, Bc);
+        bc.setup(p, Ec, Bc, dt);
+    }
     for (; it <= nt; ++it) {
         R vprev[3] = {s.v[0], s.v[1], s.v[2]};
         t += dt;                                              // :454
-        R Ec[3], Bc[3];
-        s.eval(p, t, Ec, Bc);
-        s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vt);           // :455
+        if constexpr (HOIST) {
+            bc.apply(s.v, vt);                                // :455
+        } else {
+            s.eval(p, t, Ec, Bc);
+            s.template Uf<ALG>(p, Ec, Bc, s.v, dt, vt);       // :455
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
         typename S::Pos xn;
@@ -367,6 +406,15 @@ __device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
     gbl(p.out_len)[i] = (int)iout;                                            // :545-551
     gbl(p.retcode)[i] = (iout < p.nout) ? kRetTerminated : kRetSuccess;
     return final_step;
+}
+
+template <class S, int ALG, bool REC>
+__device__ __forceinline__ long long run_fixed(const KParams& p, long long i)
+{
+    if constexpr (S::kHoist && ALG == 0) {
+        if (p.uniform_fields) return run_fixed_impl<S, ALG, REC, true>(p, i);
+    }
+    return run_fixed_impl<S, ALG, REC, false>(p, i);
 }
 
 // _adaptive_boris_single, src/boris/adaptive_boris.jl:145-266, as a
@@ -886,6 +934,10 @@ struct SoloLaunch {
     template <class S, int ALG>
     void go() const
     {
+        if constexpr (S::kHoist) {
+            hipLaunchKernelGGL((k_fixed<S, ALG>), g, b, 0, st, p);
+            return;
+        }
         // records with E, B or work columns: generic policies only (spec_class
         // declines them), in the REC instantiations
         if constexpr (S::kNS < 0) {
