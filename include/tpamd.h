@@ -146,7 +146,7 @@ typedef struct tpamd_solve_desc {
 #define TPAMD_RECORD_WIDTH(d) (6 + ((d)->save_fields ? 6 : 0) + ((d)->save_work ? 4 : 0))
 
 /* descriptor flags */
-#define TPAMD_FLAG_GENERIC_KERNELS 1  /* skip the gridded-field specialisations (for cross-checks) */
+#define TPAMD_FLAG_GENERIC_KERNELS 1  /* skip the specialised kernels (gridded B, uniform fields, the dipole): for cross-checks */
 
 typedef struct tpamd_stats {
     double kernel_ms;         /* device time of the push kernel, HIP events on the solve stream */

@@ -284,14 +284,17 @@ void select_policy(const tpamd_solve_desc* d, const KParams& p, bool allow_spec,
         }
         return;
     }
+    // analytic fields have kernels of their own for uniform fields and for
+    // the dipole (fixed-step Boris), unless the generic ones are asked for
+    const bool special = !(d->flags & TPAMD_FLAG_GENERIC_KERNELS);
     if (d->precision == TPAMD_FP64) {
         if (fk == kFkMesh) launch_generic64_mesh(r, p.alg);
         else if (fk == kFkGrid) launch_generic64(r, p.alg);
-        else launch_generic64_analytic(r, p.alg);
+        else launch_generic64_analytic(r, p.alg, special);
     } else {
         if (fk == kFkMesh) launch_generic32_mesh(r, p.alg);
         else if (fk == kFkGrid) launch_generic32(r, p.alg);
-        else launch_generic32_analytic(r, p.alg);
+        else launch_generic32_analytic(r, p.alg, special);
     }
 }
 

@@ -425,6 +425,20 @@ __host__ __device__ inline int field_class(int kind)
     return kind == kFieldMesh ? kFkMesh : ((kind == kFieldGrid || kind == kFieldTimeGrid) ? kFkGrid : kFkAnalytic);
 }
 
+// the dipole field with its moment along -z, B = b0 R^3 / r^5 (-3xz, -3yz,
+// r^2 - 3z^2): the Earth-like dipole of docs/examples/analytic/demo_proton_dipole.jl
+__device__ __forceinline__ void dipole_f64(const DevField& f, const double x[3], double o[3])
+{
+    const double r2 = dot3(x, x);
+    const double r = sqrt(r2);
+    const double r5 = (r2 * r2) * r;
+    const double k = f.b0 * ((f.radius * f.radius) * f.radius);
+    const double fac = k / r5;
+    o[0] = fac * ((-3.0 * x[0]) * x[2]);
+    o[1] = fac * ((-3.0 * x[1]) * x[2]);
+    o[2] = fac * (r2 - (3.0 * x[2]) * x[2]);
+}
+
 template <int FK = kFkMesh>
 __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], double t, double o[3])
 {
@@ -448,17 +462,9 @@ __device__ __forceinline__ void field_f64(const DevField& f, const double x[3], 
             o[c] = r + f.ft[c] * t;
         }
         return;
-    case kFieldDipole: {
-        const double r2 = dot3(x, x);
-        const double r = sqrt(r2);
-        const double r5 = (r2 * r2) * r;
-        const double k = f.b0 * ((f.radius * f.radius) * f.radius);
-        const double fac = k / r5;
-        o[0] = fac * ((-3.0 * x[0]) * x[2]);
-        o[1] = fac * ((-3.0 * x[1]) * x[2]);
-        o[2] = fac * (r2 - (3.0 * x[2]) * x[2]);
+    case kFieldDipole:
+        dipole_f64(f, x, o);
         return;
-    }
     case kFieldGrid:
         if constexpr (FK >= kFkGrid) {
             grid_f64(f, x, o);
@@ -961,6 +967,22 @@ __device__ __forceinline__ void pos_f32(const KParams& p, const Pos32& x, float 
     }
 }
 
+// dipole_f64 in float at the position of a compensated state
+template <int FK>
+__device__ __forceinline__ void dipole_f32(const KParams& p, const DevField& f, const Pos32& x, float o[3])
+{
+    float xf[3];
+    pos_f32<FK>(p, x, xf);
+    // b0 (R/r)^3 / r^2 keeps every intermediate inside float's normal range
+    const float r2 = dot3f(xf, xf);
+    const float ir = rsqrtf(r2);
+    const float q = (float)f.radius * ir;
+    const float fac = (float)f.b0 * (q * q * q) * (ir * ir);
+    o[0] = fac * (-3.0f * xf[0] * xf[2]);
+    o[1] = fac * (-3.0f * xf[1] * xf[2]);
+    o[2] = fac * fmaf(-3.0f * xf[2], xf[2], r2);
+}
+
 template <int FK = kFkMesh>
 __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, const Pos32& x,
                                           double t, float o[3])
@@ -1025,19 +1047,9 @@ __device__ __forceinline__ void field_f32(const KParams& p, const DevField& f, c
         }
         return;
     }
-    case kFieldDipole: {
-        float xf[3];
-        pos_f32<FK>(p, x, xf);
-        // b0 (R/r)^3 / r^2 keeps every intermediate inside float's normal range
-        const float r2 = dot3f(xf, xf);
-        const float ir = rsqrtf(r2);
-        const float q = (float)f.radius * ir;
-        const float fac = (float)f.b0 * (q * q * q) * (ir * ir);
-        o[0] = fac * (-3.0f * xf[0] * xf[2]);
-        o[1] = fac * (-3.0f * xf[1] * xf[2]);
-        o[2] = fac * fmaf(-3.0f * xf[2], xf[2], r2);
+    case kFieldDipole:
+        dipole_f32<FK>(p, f, x, o);
         return;
-    }
     default:
         o[0] = o[1] = o[2] = 0.0f;
         return;

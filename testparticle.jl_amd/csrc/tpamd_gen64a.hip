@@ -4,12 +4,19 @@
 
 namespace tpamd {
 
-void launch_generic64_analytic(const LaunchReq& r, int alg)
+void launch_generic64_analytic(const LaunchReq& r, int alg, bool special)
 {
     // fixed-step Boris in fields that are the same everywhere: the kernel
     // that forms the update's rotation vectors once per particle
-    if (!r.batch && !r.adaptive && alg == 0 && r.p->uniform_fields && r.p->rec_w == 6) {
+    if (special && !r.batch && !r.adaptive && alg == 0 && r.p->uniform_fields && r.p->rec_w == 6) {
         SoloLaunch{*r.p, 0, r.g, r.b, r.st}.template go<Hoisted<S64A>, 0>();
+        return;
+    }
+    // fixed-step Boris in a dipole B with no E: the kernel compiled for
+    // that pair of fields
+    if (special && !r.batch && !r.adaptive && alg == 0 && r.p->rec_w == 6 && r.p->B.kind == kFieldDipole &&
+        r.p->E.kind == kFieldZero && !r.p->e_is_uxb) {
+        hipLaunchKernelGGL((k_fixed<DipoleOnly<S64A>, 0>), r.g, r.b, 0, r.st, *r.p);
         return;
     }
     with_launcher(r, [&](const auto& l) {

@@ -100,8 +100,8 @@ struct LaunchReq {
 // generic kernels, FP64 and FP32C, per field-kind class: analytic fields
 // (tpamd_gen64a/32a.hip), + grids (tpamd_gen64/32.hip), + meshes
 // (tpamd_gen64m/32m.hip)
-void launch_generic64_analytic(const LaunchReq& r, int alg);
-void launch_generic32_analytic(const LaunchReq& r, int alg);
+void launch_generic64_analytic(const LaunchReq& r, int alg, bool special);
+void launch_generic32_analytic(const LaunchReq& r, int alg, bool special);
 void launch_generic64(const LaunchReq& r, int alg);
 void launch_generic32(const LaunchReq& r, int alg);
 void launch_generic64_mesh(const LaunchReq& r, int alg);

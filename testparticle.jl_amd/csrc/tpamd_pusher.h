@@ -209,6 +209,28 @@ struct Hoisted : Base {
     static constexpr bool kHoist = true;
 };
 
+// a policy whose fields are fixed when the kernel is compiled: B the dipole,
+// E zero (trapped particles in a planetary field).  Its kernel carries no
+// field-kind switch; B is computed by the dipole case of field_f64 /
+// field_f32 itself, so results equal the generic kernel's.
+template <class Base>
+struct DipoleOnly : Base {
+    using R = typename Base::R;
+    __device__ __forceinline__ void eval(const KParams& p, double, R E[3], R B[3]) const
+    {
+        E[0] = E[1] = E[2] = R(0);
+        if constexpr (sizeof(R) == 8) dipole_f64(p.B, this->pos.x, B);
+        else dipole_f32<Base::kFK>(p, p.B, this->pos, B);
+    }
+    template <int ALG>
+    __device__ __forceinline__ void U(const KParams& p, const R vin[3], double h, double t, R out[3]) const
+    {
+        R E[3], B[3];
+        eval(p, t, E, B);
+        this->template Uf<ALG>(p, E, B, vin, h, out);
+    }
+};
+
 template <class S>
 __device__ __forceinline__ bool outside(const KParams& p, const S& s, const typename S::Pos& xn,
                                         const typename S::R vv[3])
@@ -934,26 +956,28 @@ struct SoloLaunch {
     template <class S, int ALG>
     void go() const
     {
+        // the hoisted policies are launched for fixed-step Boris with plain
+        // records only, so no other kernel of theirs is instantiated
         if constexpr (S::kHoist) {
             hipLaunchKernelGGL((k_fixed<S, ALG>), g, b, 0, st, p);
-            return;
-        }
-        // records with E, B or work columns: generic policies only (spec_class
-        // declines them), in the REC instantiations
-        if constexpr (S::kNS < 0) {
-            if (p.rec_w > 6) {
-                if (adaptive) hipLaunchKernelGGL((k_adaptive<S, ALG, true>), g, b, 0, st, p);
-                else hipLaunchKernelGGL((k_fixed<S, ALG, true>), g, b, 0, st, p);
-                return;
+        } else {
+            // records with E, B or work columns: generic policies only
+            // (spec_class declines them), in the REC instantiations
+            if constexpr (S::kNS < 0) {
+                if (p.rec_w > 6) {
+                    if (adaptive) hipLaunchKernelGGL((k_adaptive<S, ALG, true>), g, b, 0, st, p);
+                    else hipLaunchKernelGGL((k_fixed<S, ALG, true>), g, b, 0, st, p);
+                    return;
+                }
             }
-        }
-        if constexpr (S::kNS <= 0) {
-            if (adaptive) {
-                hipLaunchKernelGGL((k_adaptive<S, ALG>), g, b, 0, st, p);
-                return;
+            if constexpr (S::kNS <= 0) {
+                if (adaptive) {
+                    hipLaunchKernelGGL((k_adaptive<S, ALG>), g, b, 0, st, p);
+                    return;
+                }
             }
+            hipLaunchKernelGGL((k_fixed<S, ALG>), g, b, 0, st, p);
         }
-        hipLaunchKernelGGL((k_fixed<S, ALG>), g, b, 0, st, p);
     }
 };
 

@@ -168,7 +168,8 @@ def solve(prob: TraceProblem, alg: AbstractBoris, *args, trajectories: int = 1,
     ratios of integer polynomials in tau^2, no transcendental),
     `max_steps` (adaptive step cap), `dts` + `nsteps` (per-particle step,
     fixed-step only), `nout_cap` (adaptive record capacity), `kernels`
-    ("auto" | "generic": skip the gridded-field specialisations),
+    ("auto" | "generic": skip the specialised kernels for gridded B, uniform
+    fields and the dipole),
     `particles_per_lane` (adaptive only: each wave of 64 lanes works through
     64 x this many particles, a lane taking the next when its own ends; 0 =
     the library's choice from N)."""

@@ -1,0 +1,76 @@
+"""The kernels compiled for one pair of analytic fields against the generic
+analytic kernels (kernels="generic"), bit for bit: fixed-step Boris in fields
+that are the same everywhere (the update's rotation vectors formed once per
+particle) and in the dipole with no E (no field-kind switch per step).  FP64
+is also held to the oracle."""
+import numpy as np
+import pytest
+
+import testparticle_amd as TP
+from testparticle_amd import workloads as W
+from parity import assert_bitwise, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_same(a, b):
+    np.testing.assert_array_equal(a.lengths, b.lengths)
+    np.testing.assert_array_equal(a.retcodes, b.retcodes)
+    np.testing.assert_array_equal(a.u_all, b.u_all)
+    np.testing.assert_array_equal(a.t_all, b.t_all)
+
+
+def both(prob, precision, **kw):
+    a = TP.solve(prob, TP.Boris(), precision=precision, **kw)
+    b = TP.solve(prob, TP.Boris(), precision=precision, kernels="generic", **kw)
+    assert_same(a, b)
+    return a
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_dipole_kernel_matches_generic(precision):
+    w = W.dipole_belt(N=2048, nsteps=3000, seed=5)
+    param = (w.q2m, TP.mi, TP.ZeroField(), TP.DipoleField(w.extra["b0"], w.extra["R"]), TP.ZeroField())
+    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), param)
+    base = dict(dts=w.dts, nsteps=w.nsteps)
+    # removal at 1 R_E, no intermediate records
+    a = both(prob, precision, save_everystep=False, isoutside=TP.InsideSphere(TP.Re), **base)
+    assert a.lengths.min() >= 1
+    # records every 250 steps, loss cone and outer sphere both removing
+    a = both(prob, precision, savestepinterval=250,
+             isoutside=TP.InsideSphere(3 * TP.Re) | TP.OutsideSphere(6 * TP.Re), **base)
+    assert (a.retcodes == 1).any() and (a.retcodes == 0).any()
+    # no boundary at all
+    both(prob, precision, savestepinterval=1000, **base)
+
+
+def test_dipole_kernel_fp64_matches_oracle():
+    w = W.dipole_belt(N=1024, nsteps=2000, seed=6)
+    param = (w.q2m, TP.mi, TP.ZeroField(), TP.DipoleField(w.extra["b0"], w.extra["R"]), TP.ZeroField())
+    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), param)
+    kw = dict(dts=w.dts, nsteps=w.nsteps, savestepinterval=100,
+              isoutside=TP.InsideSphere(3 * TP.Re) | TP.OutsideSphere(6 * TP.Re))
+    assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_uniform_kernel_matches_generic(precision):
+    w = W.uniform_gyration(N=1024)
+    B = w.extra["B"]
+    for E in (TP.ZeroField(), TP.UniformField([1e-6, -2e-6, 5e-7])):
+        prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, E, TP.UniformField(B), TP.ZeroField()))
+        both(prob, precision, dt=w.dt, savestepinterval=w.savestepinterval)
+        both(prob, precision, dt=w.dt, save_everystep=False)
+        # a box on the scale of the gyroradii (~1e5 m) that particles leave
+        # along B and across it at uneven times
+        a = both(prob, precision, dt=w.dt, savestepinterval=7,
+                 isoutside=TP.OutsideBox([-1.5e5, -1.5e5, -1.5e5], [1.5e5, 1.5e5, 1.5e5]))
+        assert len(np.unique(a.lengths)) > 10
+
+
+def test_uniform_kernel_fp64_matches_oracle():
+    w = W.uniform_gyration(N=512)
+    prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, TP.UniformField([1e-6, -2e-6, 5e-7]),
+                                            TP.UniformField(w.extra["B"]), TP.ZeroField()))
+    kw = dict(dt=w.dt, savestepinterval=w.savestepinterval)
+    assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
