@@ -187,6 +187,63 @@ __device__ __forceinline__ void boris_f64(const double v[3], const double E[3], 
     for (int c = 0; c < 3; ++c) out[c] = (vm[c] + vps[c]) + h * E[c];
 }
 
+// x / d for several x over one d, by the instruction sequence of the
+// compiler's IEEE double division (scale, reciprocal, two Newton steps on
+// it, quotient, residual, fused correction, fix-up) with the denominator's
+// half formed once.  The two scale steps return their operand unchanged
+// when d is in [1, 2^64] and |x| in [2^-512, 2^64] (no overflow or
+// underflow in any step), so there the quotient is the division's own, bit
+// for bit; callers divide plainly outside that range (zero x included).
+struct SharedDiv {
+    double d, r;
+    __device__ __forceinline__ void init(double den)
+    {
+        d = den;
+        const double r0 = __builtin_amdgcn_rcp(den);
+        const double e0 = fma(-den, r0, 1.0);
+        const double r1 = fma(r0, e0, r0);
+        const double e1 = fma(-den, r1, 1.0);
+        r = fma(r1, e1, r1);
+    }
+    __device__ __forceinline__ double div(double x) const
+    {
+        const double m = x * r;
+        const double f = fma(-d, m, x);
+        return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(f, r, m, false), d, x);
+    }
+};
+
+// boris_f64 with the three divisions of s = 2t / (1 + t.t) sharing their
+// denominator (SharedDiv); the same result
+__device__ __forceinline__ void boris_f64_sd(const double v[3], const double E[3], const double B[3],
+                                             double h, double out[3])
+{
+    double t[3], s[3], vm[3], vmt[3], vp[3], vps[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t[c] = h * B[c];
+    const double den = 1.0 + dot3(t, t);
+    const double n[3] = {2.0 * t[0], 2.0 * t[1], 2.0 * t[2]};
+    // den = 1 + t.t >= 1 (a NaN fails the test below); |2 t_c| <= 2 sqrt(den),
+    // so den <= 2^64 bounds the numerators by 2^33.  & keeps the test branch-free.
+    if ((den <= 0x1p64) & (fabs(n[0]) >= 0x1p-512) & (fabs(n[1]) >= 0x1p-512) & (fabs(n[2]) >= 0x1p-512)) {
+        SharedDiv q;
+        q.init(den);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] = q.div(n[c]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] = n[c] / den;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vm[c] = v[c] + h * E[c];
+    cross3(vm, t, vmt);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vp[c] = vm[c] + vmt[c];
+    cross3(vp, s, vps);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = (vm[c] + vps[c]) + h * E[c];
+}
+
 // boris_f64 for fields that are the same at every step: t, s and h E depend
 // only on the fields and h, so they are formed once (by the same operations,
 // so the update is bit-identical) and each step does the rest.

@@ -223,11 +223,18 @@ struct DipoleOnly : Base {
         else dipole_f32<Base::kFK>(p, p.B, this->pos, B);
     }
     template <int ALG>
+    __device__ __forceinline__ void Uf(const KParams& p, const R E[3], const R B[3], const R vin[3], double h,
+                                       R out[3]) const
+    {
+        if constexpr (sizeof(R) == 8 && ALG == 0) boris_f64_sd(vin, E, B, (p.q2m * 0.5) * h, out);
+        else Base::template Uf<ALG>(p, E, B, vin, h, out);
+    }
+    template <int ALG>
     __device__ __forceinline__ void U(const KParams& p, const R vin[3], double h, double t, R out[3]) const
     {
         R E[3], B[3];
         eval(p, t, E, B);
-        this->template Uf<ALG>(p, E, B, vin, h, out);
+        Uf<ALG>(p, E, B, vin, h, out);
     }
 };
 

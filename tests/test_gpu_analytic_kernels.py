@@ -46,8 +46,15 @@ def test_dipole_kernel_matches_generic(precision):
 
 def test_dipole_kernel_fp64_matches_oracle():
     w = W.dipole_belt(N=1024, nsteps=2000, seed=6)
+    xv0 = w.xv0.copy()
+    # every 7th particle starts on the equator (B_x = B_y = 0: the Boris
+    # divisions have zero numerators) and every 11th 1e-300 m above it (tiny
+    # numerators): their first steps take the plain-division path, lanes of
+    # the same waves the shared-denominator one
+    xv0[::7, 2] = 0.0
+    xv0[::11, 2] = 1e-300
     param = (w.q2m, TP.mi, TP.ZeroField(), TP.DipoleField(w.extra["b0"], w.extra["R"]), TP.ZeroField())
-    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), param)
+    prob = TP.TraceProblem(xv0, (0.0, 0.0), param)
     kw = dict(dts=w.dts, nsteps=w.nsteps, savestepinterval=100,
               isoutside=TP.InsideSphere(3 * TP.Re) | TP.OutsideSphere(6 * TP.Re))
     assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
