@@ -448,8 +448,8 @@ def bench_dipole(args, torch, TP, W, DeviceEnsemble):
                                          "save_everystep = false, boris.jl:523-551)"},
                      "roofline": {"bound": "valu", "achieved": flops / 1e12, "peak": peak / 1e12,
                                   "unit": "TFLOP/s", "frac": flops / peak,
-                                  "kernel": "k_fixed<S64A, Boris>" if prec == "fp64"
-                                  else "k_fixed<S32CA, Boris>"}}
+                                  "kernel": "k_fixed<DipoleOnly<S64A>, Boris>" if prec == "fp64"
+                                  else "k_fixed<DipoleOnly<S32CA>, Boris>"}}
         if args.valu:
             # against the FMA rate measured on this GPU (FP32 without packed math)
             meas = args.valu["fp64_fma_flops_per_s" if prec == "fp64" else "fp32_fma_flops_per_s"]
