@@ -387,6 +387,24 @@ __device__ __forceinline__ long long run_fixed_impl(const KParams& p, long long 
         bc.setup(p, Ec, Bc, dt);
     }
     for (; it <= nt; ++it) {
+        if constexpr (HOIST) {
+            // with no boundary, the steps before the next save step need
+            // neither the outside test nor the save test: the same update
+            // and position step, without them
+            if (!p.bnd_flags) {
+                const long long stop = next_save - 1 < nt ? next_save - 1 : nt;
+                for (; it <= stop; ++it) {
+                    t += dt;
+                    bc.apply(s.v, vt);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) s.v[c] = vt[c];
+                    typename S::Pos xn;
+                    do_step(p, s, s.v, dt, xn);
+                    s.pos = xn;
+                }
+                if (it > nt) break;
+            }
+        }
         R vprev[3] = {s.v[0], s.v[1], s.v[2]};
         t += dt;                                              // :454
         if constexpr (HOIST) {
