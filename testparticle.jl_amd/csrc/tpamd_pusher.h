@@ -169,7 +169,8 @@ struct S32CT {
             float s, e;
             two_sum(pos.hi[c], d + pos.lo[c], s, e);
             int w = pos.wrap[c];
-            if (p.frame_periodic) {
+            // analytic fields have no frame: no grid, no period (setup_fields)
+            if (FK != kFkAnalytic && p.frame_periodic) {
                 const float P = (float)p.frame_period[c];
                 if (P > 0.0f && (s >= P || s < 0.0f)) {
                     const float Plo = (float)(p.frame_period[c] - (double)P);
