@@ -1015,7 +1015,7 @@ __device__ __forceinline__ void pos_f32(const KParams& p, const Pos32& x, float 
 {
     if constexpr (FK == kFkAnalytic) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) o[d] = 0.0f + (x.hi[d] + x.lo[d]);
+        for (int d = 0; d < 3; ++d) o[d] = x.hi[d] + x.lo[d];
     } else {
         double xa[3];
         pos_abs(p, x, xa);
