@@ -60,6 +60,22 @@ def test_dipole_kernel_fp64_matches_oracle():
     assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
 
 
+@pytest.mark.parametrize("scale", [1e-160, 1e-152, 1e-100, 1.0, 1e9, 1e10, 1e11, 1e30])
+def test_dipole_kernel_fp64_across_field_magnitudes(scale):
+    # the dipole kernel's three Boris divisions share their denominator
+    # 1 + t.t while it is <= 2^64 and every 2 t_c is >= 2^-512 in magnitude,
+    # and divide plainly otherwise.  Scaling the field with the same steps
+    # moves |t| from ~1e-161 to ~1e29 (about 0.1 unscaled): 1e-160 and
+    # 1e11, 1e30 lie outside the shared range, 1e-152 (|2 t| ~ 2e-153) and
+    # 1e10 (1 + t.t ~ 1e18) just inside its edges.  Bit for bit either way.
+    w = W.dipole_belt(N=256, nsteps=200, seed=13)
+    param = (w.q2m, TP.mi, TP.ZeroField(), TP.DipoleField(w.extra["b0"] * scale, w.extra["R"]),
+             TP.ZeroField())
+    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), param)
+    kw = dict(dts=w.dts, nsteps=w.nsteps, savestepinterval=20)
+    assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
 def test_uniform_kernel_matches_generic(precision):
     w = W.uniform_gyration(N=1024)
