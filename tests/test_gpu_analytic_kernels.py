@@ -76,6 +76,30 @@ def test_dipole_kernel_fp64_across_field_magnitudes(scale):
     assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
 
 
+def test_negative_rotation_factor():
+    # h = q/m dt / 2 < 0 for electrons and for tracing backward in time: the
+    # numerators of the shared divisions and the hoisted t, s change sign.
+    # Backward in time the reference keeps no closing record at t1 (its test
+    # is tsave[end] < t_final, boris.jl:531), so those particles end
+    # Terminated with one record fewer; the device follows it.
+    w = W.dipole_belt(N=512, nsteps=300, seed=17)
+    dip = TP.DipoleField(w.extra["b0"], w.extra["R"])
+    cases = [(TP.TraceProblem(w.xv0, (0.0, 0.0), (-w.q2m * TP.mi / TP.me, TP.mi, TP.ZeroField(), dip,
+                                                  TP.ZeroField())), dict(dts=w.dts, nsteps=w.nsteps)),
+             (TP.TraceProblem(w.xv0, (0.0, 0.0), (w.q2m, TP.mi, TP.ZeroField(), dip, TP.ZeroField())),
+              dict(dts=-w.dts, nsteps=w.nsteps))]
+    g = W.uniform_gyration(N=256, nsteps=640)
+    cases.append((TP.TraceProblem(g.xv0, (g.tspan[1], g.tspan[0]),
+                                  (g.q2m, TP.mi, TP.UniformField([1e-6, 0.0, 0.0]),
+                                   TP.UniformField(g.extra["B"]), TP.ZeroField())), dict(dt=-g.dt)))
+    for prob, kw in cases:
+        kw = dict(kw, savestepinterval=50)
+        a = both(prob, "fp64", **kw)
+        assert_bitwise(a, run_oracle(prob, TP.Boris(), **kw))
+        both(prob, "fp32c", **kw)
+    assert (a.retcodes == 1).all()           # backward: no closing record
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
 def test_uniform_kernel_matches_generic(precision):
     w = W.uniform_gyration(N=1024)
