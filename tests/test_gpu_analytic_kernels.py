@@ -3,6 +3,10 @@ analytic kernels (kernels="generic"), bit for bit: fixed-step Boris in fields
 that are the same everywhere (the update's rotation vectors formed once per
 particle) and in the dipole with no E (no field-kind switch per step).  FP64
 is also held to the oracle."""
+import json
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -74,6 +78,19 @@ def test_dipole_kernel_fp64_across_field_magnitudes(scale):
     prob = TP.TraceProblem(w.xv0, (0.0, 0.0), param)
     kw = dict(dts=w.dts, nsteps=w.nsteps, savestepinterval=20)
     assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
+
+
+def test_shared_division_equals_ieee_division():
+    # SharedDiv (the dipole kernel's three divisions over one denominator)
+    # against the compiler's IEEE division on 2^29 random pairs across the
+    # range it is used in (d in [1, 2^64], |x| in [2^-512, 2^33]), edges
+    # included: not one bit may differ
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "testparticle.jl_amd",
+                       "_build", "shared_div_check")
+    r = subprocess.run([exe, "1024"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert out["pairs"] >= 5e8 and out["differing"] == 0
 
 
 def test_negative_rotation_factor():
