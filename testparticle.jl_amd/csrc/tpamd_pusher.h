@@ -621,6 +621,164 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S::kWa
     add_steps(p, steps);
 }
 
+// ---------------------------------------------------------------------------
+// Fixed-step Boris in fields that are the same everywhere, with no boundary,
+// in FP64, one particle per quad of lanes: lane c of the quad holds
+// component c of x and v (lane 3 repeats component 0 and stores nothing).
+// A solve of few particles is one chain of dependent instructions per wave;
+// split by component, each lane issues about half the instructions a step,
+// and the other components of a vector come from the quad's other lanes by
+// DPP quad permutations.  Every component is formed by the operations of
+// boris_f64 and cross3 in their order, so the results are those of
+// run_fixed_impl bit for bit.  Control flow depends on particle values only,
+// so the four lanes of a quad always execute together.
+// ---------------------------------------------------------------------------
+namespace quad {
+
+// every lane of a quad_perm reads a lane of its own quad, so no lane keeps
+// an old value and none is needed
+template <int CTRL>
+__device__ __forceinline__ double perm(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// components c+1 and c+2 (mod 3) of a vector held by the quad's lanes 0-2:
+// quad_perm [1,2,0,3] and [2,0,1,3]
+__device__ __forceinline__ double next1(double x) { return perm<0xC9>(x); }
+__device__ __forceinline__ double next2(double x) { return perm<0xD2>(x); }
+__device__ __forceinline__ double pick(const double a[3], int c) { return c == 0 ? a[0] : (c == 1 ? a[1] : a[2]); }
+
+// boris_f64 with rotation factor h in uniform fields, for component c
+struct Rot {
+    double t1, t2, s1, s2, hE;    // t_{c+1}, t_{c+2}, s_{c+1}, s_{c+2}, h E_c
+    __device__ __forceinline__ void init(const double E[3], const double B[3], double h, int c)
+    {
+        BorisConst64 k;
+        k.init(E, B, h);
+        const int c1 = c == 2 ? 0 : c + 1, c2 = c == 0 ? 2 : c - 1;
+        t1 = pick(k.t, c1);
+        t2 = pick(k.t, c2);
+        s1 = pick(k.s, c1);
+        s2 = pick(k.s, c2);
+        hE = pick(k.hE, c);
+    }
+    __device__ __forceinline__ double apply(double v) const
+    {
+        const double vm = v + hE;
+        const double vmt = next1(vm) * t2 - next2(vm) * t1;   // cross3(vm, t)_c
+        const double vp = vm + vmt;
+        const double vps = next1(vp) * s2 - next2(vp) * s1;   // cross3(vp, s)_c
+        return (vm + vps) + hE;
+    }
+};
+
+// the fields of S64T::fields for zero / uniform kinds
+__device__ __forceinline__ void uniform_fields(const KParams& p, double E[3], double B[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) B[k] = p.B.kind == kFieldUniform ? p.B.v[k] : 0.0;
+    if (p.e_is_uxb) {
+        cross3(B, p.E.v, E);                                  // E = -u x B = B x u
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) E[k] = p.E.kind == kFieldUniform ? p.E.v[k] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void put(const KParams& p, long long i, long long k, int c, bool owner, bool lead,
+                                    double x, double v, double t)
+{
+    const auto r = gbl(p.out_u) + (i * p.cap + k) * 6;
+    if (owner) {
+        r[c] = x;
+        r[3 + c] = v;
+    }
+    if (lead) gbl(p.out_t)[i * p.cap + k] = t;
+}
+
+// run_fixed_impl<S64T, Boris, plain records, HOIST> with bnd_flags == 0
+__device__ __forceinline__ long long run(const KParams& p, long long i, int c, bool owner, bool lead)
+{
+    const auto xv = gbl(p.xv0) + 6 * i;
+    double x = xv[c], v = xv[3 + c];
+    const double dt = p.dt_pp ? gbl(p.dt_pp)[i] : p.dt;
+    const double t0 = p.t0;
+    const double t1 = p.dt_pp ? t0 + (double)p.nt * dt : p.t1;
+    double E[3], B[3];
+    uniform_fields(p, E, B);
+    long long iout = 0;
+    double tlast = 0.0;
+    if (p.save_start) {                                       // boris.jl:507-511
+        put(p, i, 0, c, owner, lead, x, v, t0);
+        tlast = t0;
+        iout = 1;
+    }
+    Rot r;
+    r.init(E, B, (p.q2m * 0.5) * (-0.5 * dt), c);             // :514
+    v = r.apply(v);
+    const long long nt = p.nt;
+    long long it = 1;
+    long long next_save = p.save_every ? p.K + 1 : nt + 2;
+    double t = fma(-0.5, dt, t0);                             // :451
+    Rot step, half;
+    step.init(E, B, (p.q2m * 0.5) * dt, c);
+    half.init(E, B, (p.q2m * 0.5) * (0.5 * dt), c);
+    while (it <= nt) {
+        const long long stop = next_save - 1 < nt ? next_save - 1 : nt;
+        for (; it <= stop; ++it) {                            // :454-457, no save due
+            t += dt;
+            v = step.apply(v);
+            x = x + v * dt;
+        }
+        if (it > nt) break;
+        const double vprev = v;                               // it == next_save
+        t += dt;
+        v = step.apply(v);
+        const double xn = x + v * dt;
+        next_save += p.K;
+        iout += 1;
+        if (iout <= p.nout) {                                 // :463-472
+            const double tc = fma(-0.5, dt, t);
+            put(p, i, iout - 1, c, owner, lead, x, half.apply(vprev), tc);
+            tlast = tc;
+        }
+        x = xn;
+        ++it;
+    }
+    const long long final_step = it - 1;
+    const bool should = p.save_end || (p.save_every && final_step > 0 && final_step % p.K == 0);
+    if (iout < p.nout && should) {                            // :522-543
+        const double tf = (final_step == nt) ? t1 : fma((double)final_step, dt, t0);
+        if (iout == 0 || tlast < tf) {
+            const double dtf = tf - fma((double)final_step - 0.5, dt, t0);
+            r.init(E, B, (p.q2m * 0.5) * dtf, c);
+            put(p, i, iout, c, owner, lead, x, r.apply(v), tf);
+            iout += 1;
+        }
+    }
+    if (lead) {                                               // :545-551
+        gbl(p.out_len)[i] = (int)iout;
+        gbl(p.retcode)[i] = (iout < p.nout) ? kRetTerminated : kRetSuccess;
+    }
+    return lead ? final_step : 0;
+}
+
+}  // namespace quad
+
+// four threads per particle: threadIdx 4 i + c
+__global__ void __launch_bounds__(256) k_fixed_quad(const KParams p)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = g >> 2;
+    const int lc = (int)(g & 3);
+    long long steps = 0;
+    if (i < p.N) steps = quad::run(p, i, lc == 3 ? 0 : lc, lc < 3, lc == 0);
+    add_steps(p, steps);
+}
+
 // Adaptive particles [begin, end) of one problem on one wave: lives are
 // uneven (a particle may leave after one step or run to the step cap), so
 // instead of giving each lane a fixed list, which holds the wave until the
