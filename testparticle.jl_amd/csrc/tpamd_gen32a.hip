@@ -6,8 +6,17 @@ namespace tpamd {
 
 void launch_generic32_analytic(const LaunchReq& r, int alg, bool special)
 {
-    // fixed-step Boris in fields that are the same everywhere: the kernel
-    // that forms the update's rotation vectors once per particle
+    // fixed-step Boris in fields that are the same everywhere: with no
+    // boundary, one particle per quad of lanes (four threads a particle);
+    // else the kernel that forms the update's rotation vectors once per
+    // particle
+    if (special && !r.batch && !r.adaptive && alg == 0 && r.p->uniform_fields && r.p->rec_w == 6 &&
+        r.p->bnd_flags == 0) {
+        const long long threads = 4 * r.p->N;
+        const dim3 g((unsigned)((threads + r.b.x - 1) / r.b.x));
+        hipLaunchKernelGGL(k_fixed_quad32, g, r.b, 0, r.st, *r.p);
+        return;
+    }
     if (special && !r.batch && !r.adaptive && alg == 0 && r.p->uniform_fields && r.p->rec_w == 6) {
         SoloLaunch{*r.p, 0, r.g, r.b, r.st}.template go<Hoisted<S32CA>, 0>();
         return;

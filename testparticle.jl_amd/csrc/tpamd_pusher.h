@@ -766,6 +766,140 @@ __device__ __forceinline__ long long run(const KParams& p, long long i, int c, b
     return lead ? final_step : 0;
 }
 
+// FP32C: boris_f32 / BorisConst32 for component c, and the compensated
+// position step of S32CT::step_impl for analytic fields (no frame)
+__device__ __forceinline__ float permf(float x, int which)
+{
+    const int b = __float_as_int(x);
+    return __int_as_float(which == 1 ? __builtin_amdgcn_mov_dpp(b, 0xC9, 0xF, 0xF, true)
+                                     : __builtin_amdgcn_mov_dpp(b, 0xD2, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float pickf(const float a[3], int c) { return c == 0 ? a[0] : (c == 1 ? a[1] : a[2]); }
+
+struct Rot32 {
+    float t1, t2, s1, s2, e, h;   // t_{c+1}, t_{c+2}, s_{c+1}, s_{c+2}, E_c, h
+    __device__ __forceinline__ void init(const float E[3], const float B[3], float hv, int c)
+    {
+        BorisConst32 k;
+        k.init(E, B, hv);
+        const int c1 = c == 2 ? 0 : c + 1, c2 = c == 0 ? 2 : c - 1;
+        t1 = pickf(k.t, c1);
+        t2 = pickf(k.t, c2);
+        s1 = pickf(k.s, c1);
+        s2 = pickf(k.s, c2);
+        e = pickf(k.E, c);
+        h = k.h;
+    }
+    __device__ __forceinline__ float apply(float v) const
+    {
+        const float vm = fmaf(h, e, v);
+        const float vmt = fmaf(permf(vm, 1), t2, -(permf(vm, 2) * t1));   // cross3f(vm, t)_c
+        const float vp = vm + vmt;
+        const float vps = fmaf(permf(vp, 1), s2, -(permf(vp, 2) * s1));   // cross3f(vp, s)_c
+        return fmaf(h, e, vm + vps);
+    }
+};
+
+// S32CT::fields for zero / uniform kinds, in float
+__device__ __forceinline__ void uniform_fields32(const KParams& p, float E[3], float B[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) B[k] = p.B.kind == kFieldUniform ? (float)p.B.v[k] : 0.0f;
+    if (p.e_is_uxb) {
+        const float u[3] = {(float)p.E.v[0], (float)p.E.v[1], (float)p.E.v[2]};
+        cross3f(B, u, E);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) E[k] = p.E.kind == kFieldUniform ? (float)p.E.v[k] : 0.0f;
+    }
+}
+
+// record values of S32CT with no frame: 0 + (hi + lo) in double, (double) v
+__device__ __forceinline__ void put32(const KParams& p, long long i, long long k, int c, bool owner, bool lead,
+                                      float hi, float lo, float v, double t)
+{
+    put(p, i, k, c, owner, lead, 0.0 + ((double)hi + (double)lo), (double)v, t);
+}
+
+// run_fixed_impl<S32CT<analytic>, Boris, plain records, HOIST> with
+// bnd_flags == 0
+__device__ __forceinline__ long long run32(const KParams& p, long long i, int c, bool owner, bool lead)
+{
+    const auto xv = gbl(p.xv0) + 6 * i;
+    const double r0 = xv[c] - p.frame_origin[c];              // S32CT::load_from (no period)
+    float hi = (float)r0;
+    float lo = (float)(r0 - (double)hi);
+    float v = (float)xv[3 + c];
+    const double dt = p.dt_pp ? gbl(p.dt_pp)[i] : p.dt;
+    const float dt_hi = (float)dt;                            // S32CT::set_dt
+    const float dt_lo = (float)(dt - (double)dt_hi);
+    const double t0 = p.t0;
+    const double t1 = p.dt_pp ? t0 + (double)p.nt * dt : p.t1;
+    float E[3], B[3];
+    uniform_fields32(p, E, B);
+    long long iout = 0;
+    double tlast = 0.0;
+    if (p.save_start) {
+        put32(p, i, 0, c, owner, lead, hi, lo, v, t0);
+        tlast = t0;
+        iout = 1;
+    }
+    Rot32 r;
+    r.init(E, B, (float)((p.q2m * 0.5) * (-0.5 * dt)), c);
+    v = r.apply(v);
+    const long long nt = p.nt;
+    long long it = 1;
+    long long next_save = p.save_every ? p.K + 1 : nt + 2;
+    double t = fma(-0.5, dt, t0);
+    Rot32 step, half;
+    step.init(E, B, (float)((p.q2m * 0.5) * dt), c);
+    half.init(E, B, (float)((p.q2m * 0.5) * (0.5 * dt)), c);
+    auto move = [&](float vv, float& h2, float& l2) {          // S32CT::step_impl, no frame
+        const float d = fmaf(vv, dt_hi, vv * dt_lo);
+        two_sum(hi, d + lo, h2, l2);
+    };
+    while (it <= nt) {
+        const long long stop = next_save - 1 < nt ? next_save - 1 : nt;
+        for (; it <= stop; ++it) {
+            t += dt;
+            v = step.apply(v);
+            move(v, hi, lo);
+        }
+        if (it > nt) break;
+        const float vprev = v;
+        t += dt;
+        v = step.apply(v);
+        float hn, ln;
+        move(v, hn, ln);
+        next_save += p.K;
+        iout += 1;
+        if (iout <= p.nout) {
+            const double tc = fma(-0.5, dt, t);
+            put32(p, i, iout - 1, c, owner, lead, hi, lo, half.apply(vprev), tc);
+            tlast = tc;
+        }
+        hi = hn;
+        lo = ln;
+        ++it;
+    }
+    const long long final_step = it - 1;
+    const bool should = p.save_end || (p.save_every && final_step > 0 && final_step % p.K == 0);
+    if (iout < p.nout && should) {
+        const double tf = (final_step == nt) ? t1 : fma((double)final_step, dt, t0);
+        if (iout == 0 || tlast < tf) {
+            const double dtf = tf - fma((double)final_step - 0.5, dt, t0);
+            r.init(E, B, (float)((p.q2m * 0.5) * dtf), c);
+            put32(p, i, iout, c, owner, lead, hi, lo, r.apply(v), tf);
+            iout += 1;
+        }
+    }
+    if (lead) {
+        gbl(p.out_len)[i] = (int)iout;
+        gbl(p.retcode)[i] = (iout < p.nout) ? kRetTerminated : kRetSuccess;
+    }
+    return lead ? final_step : 0;
+}
+
 }  // namespace quad
 
 // four threads per particle: threadIdx 4 i + c
@@ -776,6 +910,16 @@ __global__ void __launch_bounds__(256) k_fixed_quad(const KParams p)
     const int lc = (int)(g & 3);
     long long steps = 0;
     if (i < p.N) steps = quad::run(p, i, lc == 3 ? 0 : lc, lc < 3, lc == 0);
+    add_steps(p, steps);
+}
+
+__global__ void __launch_bounds__(256) k_fixed_quad32(const KParams p)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = g >> 2;
+    const int lc = (int)(g & 3);
+    long long steps = 0;
+    if (i < p.N) steps = quad::run32(p, i, lc == 3 ? 0 : lc, lc < 3, lc == 0);
     add_steps(p, steps);
 }
 
