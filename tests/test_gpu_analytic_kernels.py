@@ -134,7 +134,10 @@ def test_uniform_kernel_matches_generic(precision):
 
 def test_uniform_kernel_fp64_matches_oracle():
     w = W.uniform_gyration(N=512)
-    prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, TP.UniformField([1e-6, -2e-6, 5e-7]),
-                                            TP.UniformField(w.extra["B"]), TP.ZeroField()))
-    kw = dict(dt=w.dt, savestepinterval=w.savestepinterval)
-    assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
+    B = TP.UniformField(w.extra["B"])
+    # uniform E, and E = -u x B of the uniform B (both one quad of lanes a particle)
+    for E in (TP.UniformField([1e-6, -2e-6, 5e-7]), TP.UxBField([1e2, -3e1, 5e1], B)):
+        prob = TP.TraceProblem(w.xv0, w.tspan, (w.q2m, TP.mi, E, B, TP.ZeroField()))
+        kw = dict(dt=w.dt, savestepinterval=w.savestepinterval)
+        assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
+        both(prob, "fp32c", **kw)
