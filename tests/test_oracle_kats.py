@@ -145,6 +145,27 @@ def test_save_flags(kats, orc):
             assert t[-2] == pytest.approx(t0 + cs["t_penultimate_steps"] * dt)
 
 
+def test_backward_tracing_keeps_no_closing_record(kats, orc):
+    # No reference test traces Boris backward; boris.jl:523-551 decides it.
+    # With t1 < t0 and dt < 0 the closing record is taken only if
+    # tsave[iout] < t_final, which never holds going backward, so the
+    # solution has one record fewer than nout and the retcode is Terminated;
+    # the same problem forward keeps the record at t1 and ends Success.
+    k = kats["basic_boris"]
+    q2m = q2m_of(kats, "electron")
+    t0, t1 = k["tspan"]
+    K = 10
+    nout = orc.calculate_nout(1000, K, True, True, True)
+    fwd = orc.solve(k["u0"], orc.zero(), orc.uniform(k["B"]), q2m=q2m, tspan=(t0, t1), dt=k["dt"],
+                    savestepinterval=K)
+    back = orc.solve(k["u0"], orc.zero(), orc.uniform(k["B"]), q2m=q2m, tspan=(t1, t0), dt=-k["dt"],
+                     savestepinterval=K)
+    assert fwd.len[0] == nout and fwd.retcode[0] == 0 and fwd.t[0, nout - 1] == t1
+    assert back.len[0] == nout - 1 and back.retcode[0] == 1
+    tb = back.t[0, :nout - 1]
+    assert tb[0] == t1 and np.all(np.diff(tb) < 0) and tb[-1] > t0
+
+
 def test_kernel_path_energy_and_saving(kats, orc):
     k = kats["kernel_path"]
     c = kats["constants"]
