@@ -141,3 +141,9 @@ def test_uniform_kernel_fp64_matches_oracle():
         kw = dict(dt=w.dt, savestepinterval=w.savestepinterval)
         assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
         both(prob, "fp32c", **kw)
+    # a step of its own per particle (dts with nsteps): each particle's t1 differs
+    dts = w.dt * np.random.default_rng(3).uniform(0.5, 1.5, len(w.xv0))
+    prob = TP.TraceProblem(w.xv0, (0.0, 0.0), (w.q2m, TP.mi, TP.ZeroField(), B, TP.ZeroField()))
+    kw = dict(dts=dts, nsteps=700, savestepinterval=30)
+    assert_bitwise(TP.solve(prob, TP.Boris(), **kw), run_oracle(prob, TP.Boris(), **kw))
+    both(prob, "fp32c", **kw)
